@@ -1,0 +1,187 @@
+"""Benchmark: SR training epochs of the toric code 4x4 (N = 32 spins),
+basic complex RBM alpha = 2 (P = 32 + 64 + 2048 = 2144), 4096 Metropolis
+chains x 64 kept samples (262 144 samples per epoch), 100 warm-up sweeps,
+bond_flips 0.5, matrix-free SR solved by CG with FP64 vectors
+(sr.max_iterations = 100 as in examples/toric/toric.ini).
+
+One step = one full epoch: sample (persistent chain kernel with fused local
+energies) -> O assembly -> reduction -> F -> CG solve -> weight update, all
+on one GPU.  Inputs (weights, tables) are resident in HBM before timing.
+
+Prints one JSON line (see the driver contract) with `roofline` for the
+dominant kernel (the S.x product of the CG) and `cpu_baseline` (the FP64
+CPU oracle on a bounded sample of the same workload, rank 0 only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rbm-nqs_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "samples/s and SR epoch ms at fixed energy error, plain vs learned proposal"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(model="toric", n_cells=4, rbm="basic", alpha=2, chains=4096, samples_per_chain=64,
+                warmup=100, bond_flips=0.5, sr_max_iterations=100, precision=32)
+
+
+def spec(chains, samples_per_chain, warmup=100, max_iter=100):
+    return f"""
+[model]
+type = toric
+n_cells = 4
+J = -1
+[rbm]
+type = basic
+alpha = 2
+[sampler]
+n_samples = {chains * samples_per_chain}
+metropolis.n_chains = {chains}
+metropolis.n_warmup_steps = {warmup}
+metropolis.n_steps_per_sample = 1
+metropolis.bond_flips = 0.5
+[optimizer]
+learning_rate = 0.01
+sr.max_iterations = {max_iter}
+"""
+
+
+class Decay:
+    """decay_t of abstract_optimizer.cpp:27-49 (decays before returning)."""
+
+    def __init__(self, initial, minimum, decay):
+        self.v, self.min, self.decay, self.done = initial, minimum, decay, decay == 1.0
+
+    def get(self):
+        if not self.done:
+            self.v *= self.decay
+            if self.v < self.min:
+                self.v, self.done = self.min, True
+        return self.v
+
+
+def gpu_epochs(steps, warmup):
+    import rbmgpu
+
+    m = rbmgpu.Model(spec(WORKLOAD["chains"], WORKLOAD["samples_per_chain"]))
+    g = m.gpu(precision=WORKLOAD["precision"])
+    g.set_params(m.initial_weights(421390484))
+    lr, k1, k2, kd = Decay(1e-2, 1e-2, 1.0), Decay(1, 1e-4, 0.9), Decay(1e-3, 1e-6, 0.9), Decay(1e-2, 1e-2, 1.0)
+    dw = np.zeros(g.P, dtype=np.complex128)
+    log = []
+
+    def epoch(e):
+        nonlocal dw
+        g.sample_epoch(epoch=e)
+        r = g.reduce()
+        F = r["eo"] - r["mean"] * r["o_mean"].conj()
+        rate = lr.get()
+        r1, r2, rd = k1.get(), k2.get(), kd.get()
+        dw, itn, rs = g.sr_solve(F, dw, r1, r2, rd, max_iter=WORKLOAD["sr_max_iterations"])
+        dw *= rate
+        g.update_weights(dw)
+        log.append(dict(E=r["mean"].real / m.N, std=r["std"] / m.N, acc=r["acc"], tau=r["tau"], itn=itn))
+
+    for e in range(warmup):
+        epoch(e)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for e in range(warmup, warmup + steps):
+        epoch(e)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    times = g.kernel_times()
+    return m, g, dt, log, times
+
+
+def cpu_baseline():
+    """FP64 CPU oracle epoch (same model, bounded: 64 chains x 32 samples)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    chains, spc = 64, 32
+    o = po.Oracle(spec(chains, spc))
+    rng = np.random.default_rng(421390484)
+    o.set_params(1e-4 * (rng.normal(size=o.P) + 1j * rng.normal(size=o.P)))
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    st = o.epoch(seed=421390484, epoch=0, threads=threads, use_lut=True)
+    dt = time.perf_counter() - t0
+    S = chains * spc
+    return dict(value=S / dt, unit="samples/s", cores=threads, kind="port",
+                sample=f"1 SR epoch of the same model with {chains} chains x {spc} samples "
+                       f"({S} samples, 100 warm-up sweeps, CG 100 iterations); FP64 oracle with the "
+                       f"reference's lncosh cache, chains on {threads} OpenMP threads, CG on 1 thread; "
+                       f"sampling {st['t_sample']:.2f} s, SR {st['t_opt']:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one GPU, and the "
+              "multi-GPU path is a design (chains shard across GPUs), not a run.", file=sys.stderr)
+        sys.exit(2)
+    m, g, dt, log, times = gpu_epochs(a.steps, a.warmup)
+    S = WORKLOAD["chains"] * WORKLOAD["samples_per_chain"]
+    ms = 1e3 * dt / a.steps
+    value = S * a.steps / dt
+    # dominant kernel: S.x = one pass over the FP32 O matrix (P x S complex)
+    sx_ms = times["sx"]
+    o_bytes = g.P * S * 8
+    achieved = o_bytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
+    last = log[-1]
+    tau = float(np.mean([l["tau"] for l in log[a.warmup:]]))
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "toric code 4x4 (N=32), basic RBM alpha=2 (P=2144), 4096 chains x 64 samples, "
+                        "SR+CG (100 iterations, FP64 vectors); one step = one SR epoch",
+            "samples_per_epoch": S,
+            "parallelism": "single GPU",
+        },
+        "effective_samples_per_s": value / (1 + 2 * max(tau, 0.0)),
+        "tau_int": tau,
+        "energy_per_site": last["E"],
+        "energy_stddev_per_site": last["std"],
+        "acceptance": last["acc"],
+        "cg_iterations": last["itn"],
+        "kernel_ms": times,
+        "launch_shape": list(g.launch_shape()),
+        "roofline": {
+            "kernel": "k_sx_partial + k_sx_finish (one S.x of the CG)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/*
+ * rbmgpu.h -- C ABI of the MI355X (gfx950) library for the variational
+ * Monte Carlo epoch of rbm-nqs: RBM (+ Pfaffian) amplitudes, Metropolis
+ * chains, local energies, O matrix, aggregation and the stochastic-
+ * reconfiguration solve, all resident on one GPU.
+ *
+ * The reference (dbucher97/rbm-nqs) has no FFI: its hot path sits behind
+ * C++ virtual interfaces.  Each entry point below names the reference
+ * interface it replaces; the host adapters in rbm-nqs_amd/host implement
+ * those interfaces on top of this ABI.
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles on the host side;
+ *   - a spin state is n_words = ceil(N/64) uint64 words, bit i set <=>
+ *     sigma_i = -1 (spin_state.cpp:44-50);
+ *   - parameter vectors follow rbm_base.cpp:117-131 / pfaffian.cpp:417-435:
+ *     [v_bias (n_vb) | h_bias (n_alpha) | W (N x n_alpha, column-major) |
+ *      Pfaffian fs_ (pf_rows x 4, column-major)];
+ *   - the O matrix is P x S column-major (one column per sample), like
+ *     outer_aggregator_lazy (aggregator.cpp:270-280);
+ *   - every function returns 0 on success and a negative code on failure;
+ *     rbmgpu_last_error() gives the message.  No exceptions cross the ABI.
+ *   - one context owns one HIP stream on the current device; a context is
+ *     not thread-safe.
+ */
+#ifndef RBMGPU_H
+#define RBMGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rbmgpu_ctx rbmgpu_ctx;
+
+enum { RBMGPU_FP32 = 32, RBMGPU_FP64 = 64 };
+
+typedef struct rbmgpu_config {
+    /* ---- machine (rbm_base.cpp:38-52, rbm_symmetry.cpp:31-46) */
+    int32_t n_spins;       /* N <= 128 (spin_state.hpp:20) */
+    int32_t n_vb;          /* visible-bias size */
+    int32_t n_alpha;       /* hidden rows n_alpha_ */
+    int32_t n_symm;        /* symmetry permutations |G| (1 = basic RBM) */
+    const int32_t* perm;   /* n_symm x N: perm[s*N+f] = indices()(f) */
+    const int32_t* basis;  /* n_vb symmetry-basis sites (construct_symm_basis) */
+    /* ---- Pfaffian factor (pfaffian.cpp:31-110); pf_rows = 0: none */
+    int32_t pf_rows;
+    const int32_t* pf_bs;  /* N x N column-major index map bs_ */
+    int32_t pf_refresh;    /* sampler.pfaffian_refresh (abstract_sampler.cpp:103) */
+    /* ---- observables: local operators (local_op.cpp:31-60) */
+    int32_t n_terms;
+    const int32_t* term_k;     /* sites per term, 1..8 */
+    const int32_t* term_sites; /* concatenated site lists */
+    const double* term_mat;    /* concatenated dense 2^k x 2^k row-major complex */
+    const int32_t* term_group; /* observable index per term, non-decreasing;
+                                  group 0 is the Hamiltonian */
+    int32_t n_groups;
+    /* ---- Metropolis sampler (metropolis_sampler.cpp:41-53) */
+    int32_t n_chains;
+    int64_t n_samples;
+    int32_t n_warmup;
+    int32_t step_size;
+    double bond_flips;
+    int32_t n_prop_bonds;
+    const int32_t* prop_bonds; /* 2 x n_prop_bonds: (a, b) of non-z bonds */
+    uint64_t seed;
+    /* ---- execution */
+    int32_t precision;       /* RBMGPU_FP32 or RBMGPU_FP64 */
+    int32_t lanes_per_chain; /* 0: autotune */
+    int32_t keep_o;          /* materialise the O matrix (needed by
+                                rbmgpu_get_samples O and the dense S.x) */
+} rbmgpu_config;
+
+typedef struct rbmgpu_moments {
+    double e_mean[2];   /* <E_loc> */
+    double e_var;       /* unbiased variance (aggregator.cpp:145-152) */
+    double e_var_binned;
+    double e_tau;       /* binned tau (aggregator.cpp:120-128) */
+    double e_std;       /* aggregator::get_stddev with binning */
+    int64_t n_bins;
+    double acceptance;  /* metropolis_sampler.cpp:89-92 */
+} rbmgpu_moments;
+
+/* metropolis_sampler / rbm_* / pfaffian constructors (main.cpp:153-239) */
+int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out);
+void rbmgpu_destroy(rbmgpu_ctx* ctx);
+const char* rbmgpu_last_error(const rbmgpu_ctx* ctx);
+/* abstract_machine::get_n_params / get_n_neural_params (abstract_machine.hpp:86-101) */
+int rbmgpu_n_params(const rbmgpu_ctx* ctx, int64_t* n_params, int64_t* n_neural);
+/* chosen launch shape: [lanes per chain, units per lane, block] */
+int rbmgpu_launch_shape(const rbmgpu_ctx* ctx, int32_t* out3);
+
+/* rbm_base::initialize_weights / load (rbm_base.cpp:58-113, 249-288) */
+int rbmgpu_set_params(rbmgpu_ctx* ctx, const double* params);
+/* rbm_base::save (rbm_base.cpp:219-247) */
+int rbmgpu_get_params(const rbmgpu_ctx* ctx, double* params);
+/* rbm_base::update_weights: w -= dw (rbm_base.cpp:117-143) */
+int rbmgpu_update_weights(rbmgpu_ctx* ctx, const double* dw);
+
+/* metropolis_sampler::sample + evaluate_and_aggregate for every kept sample
+ * (metropolis_sampler.cpp:55-116, abstract_sampler.cpp:78-101): runs all
+ * chains of one epoch on the GPU; states, local energies, tanh(theta) and
+ * (keep_o) the O matrix stay on the device. */
+int rbmgpu_sample_epoch(rbmgpu_ctx* ctx, uint32_t epoch);
+/* operate on a host-given sample set instead (parity hook): evaluates
+ * local observables, theta and O for these states as if they were sampled */
+int rbmgpu_load_samples(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n);
+
+/* aggregator::finalize for the registered aggregators of SR
+ * (abstract_optimizer.cpp:184-192, SR.cpp:74-79): E statistics, <O>,
+ * Var(O), <E O*> and the means of the other observable groups.  Array
+ * outputs may be NULL; sizes 2P, P, 2P, 2*n_groups. */
+int rbmgpu_reduce(rbmgpu_ctx* ctx, rbmgpu_moments* out, double* o_mean,
+                  double* o_var, double* eo_mean, double* group_means);
+
+/* abstract_solver::solve via cg_solver::solve (cg_solver.cpp:260-330):
+ * solves (S + reg) x = b with S from the last sampled / loaded set.
+ * x holds the warm start on entry.  rtol = 0 -> 1e-16, max_iter = 0 ->
+ * 2P (cg_solver.cpp:48-60).  method 0 = matrix-free CG on the device. */
+int rbmgpu_sr_solve(rbmgpu_ctx* ctx, const double* b, double* x, double r1,
+                    double r2, double rd, int64_t max_iter, double rtol,
+                    int64_t* itn, double* rs);
+/* one S.x product (the Aprod lambda of cg_solver.cpp:301-317) */
+int rbmgpu_apply_S(rbmgpu_ctx* ctx, const double* x, double* y, double r1,
+                   double r2, double rd);
+
+/* parity hooks on host-given states */
+int rbmgpu_eval_logpsi(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n,
+                       double* logpsi);
+/* abstract_machine::psi_over_psi (abstract_machine.hpp:197-214) */
+int rbmgpu_psi_ratio(rbmgpu_ctx* ctx, const uint64_t* state, const int32_t* flips,
+                     int32_t n_flips, double* ratio);
+/* copies of the current sample set: states (S x n_words), local
+ * observables (S x n_groups complex), O (P x S complex; needs keep_o),
+ * per-chain acceptance (n_chains).  Any pointer may be NULL. */
+int rbmgpu_get_samples(const rbmgpu_ctx* ctx, uint64_t* states, double* obs,
+                       double* O, double* chain_acc);
+int rbmgpu_n_samples(const rbmgpu_ctx* ctx, int64_t* n);
+
+/* device time of the last launch of each kernel family, from HIP events
+ * on the context's stream: [sample, obs/O assembly, reduce, S.x (mean per
+ * CG matvec), cg vector update] in milliseconds. */
+int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms5);
+int rbmgpu_synchronize(rbmgpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

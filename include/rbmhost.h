@@ -1,0 +1,50 @@
+/*
+ * rbmhost.h -- C ABI of the host layer above rbmgpu.h: builds lattice,
+ * Hamiltonian and machine tables from the reference's .ini options
+ * (src/tools/ini.cpp) the way src/main.cpp:125-239 does, and creates the
+ * GPU context from them.  Used by language bindings (ctypes) and by the
+ * `rbm` driver.
+ */
+#ifndef RBMHOST_H
+#define RBMHOST_H
+#include <stdint.h>
+
+#include "rbmgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rbmhost_model rbmhost_model;
+
+/* .ini text with the reference's keys; NULL on error (message in err) */
+rbmhost_model* rbmhost_create(const char* ini_text, char* err, int32_t err_len);
+void rbmhost_destroy(rbmhost_model* m);
+
+/* [N, P, n_neural, n_vb, n_alpha, n_symm, pf_rows, n_terms, n_groups,
+ *  n_bonds, n_chains, n_samples, n_warmup, step_size, n_prop_bonds] */
+int rbmhost_info(const rbmhost_model* m, int64_t* out15);
+int rbmhost_bonds(const rbmhost_model* m, int64_t* out /* 3 x n_bonds */);
+int rbmhost_perms(const rbmhost_model* m, int64_t* out /* n_symm x N */);
+/* sites (k each, concatenated) and 2^k x 2^k matrices of the terms */
+int rbmhost_terms(const rbmhost_model* m, int64_t* k, int64_t* sites, double* mats, int64_t* groups);
+double rbmhost_exact_energy(const rbmhost_model* m);
+
+/* switch observables to the --evaluate set (main.cpp:640-680): helper
+ * terms removed, plus one group per hexagon plaquette (Kitaev) */
+int rbmhost_use_evaluation(rbmhost_model* m, int32_t hexagons);
+/* sampler.eval_samples: n_samples = n, warmup rescaled (metropolis_sampler.cpp:289-293) */
+int rbmhost_set_n_samples(rbmhost_model* m, int64_t n);
+
+/* weights drawn like rbm_base::initialize_weights (std::mt19937(seed),
+ * normal_distribution) followed by the Pfaffian weights */
+int rbmhost_initial_weights(const rbmhost_model* m, uint64_t seed, double* params);
+
+/* GPU context for this model; precision / lanes / chains / samples of 0
+ * keep the .ini values */
+int rbmhost_gpu_create(const rbmhost_model* m, int32_t precision, int32_t lanes, rbmgpu_ctx** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

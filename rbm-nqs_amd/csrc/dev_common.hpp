@@ -1,0 +1,196 @@
+// dev_common.hpp -- device helpers shared by the gfx950 kernels: complex
+// arithmetic, precision-dispatched math, the Philox-4x32-10 counter RNG and
+// sub-wavefront group reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rbmgpu {
+
+template <typename R>
+struct cplx {
+    R re, im;
+};
+
+template <typename R>
+__host__ __device__ inline cplx<R> mk(R a, R b) {
+    return cplx<R>{a, b};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator+(cplx<R> a, cplx<R> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator-(cplx<R> a, cplx<R> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(R s, cplx<R> a) {
+    return {s * a.re, s * a.im};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> conj(cplx<R> a) {
+    return {a.re, -a.im};
+}
+template <typename R>
+__host__ __device__ inline R norm2(cplx<R> a) {
+    return a.re * a.re + a.im * a.im;
+}
+template <typename R, typename S>
+__host__ __device__ inline cplx<R> cvt(cplx<S> a) {
+    return {(R)a.re, (R)a.im};
+}
+
+// ------------------------------------------------------------------ math
+template <typename R>
+struct M;
+template <>
+struct M<double> {
+    static __device__ double exp(double x) { return ::exp(x); }
+    static __device__ double log(double x) { return ::log(x); }
+    static __device__ double log1p(double x) { return ::log1p(x); }
+    static __device__ double atan2(double y, double x) { return ::atan2(y, x); }
+    static __device__ void sincos(double x, double* s, double* c) { ::sincos(x, s, c); }
+    static __device__ double fabs(double x) { return ::fabs(x); }
+    static __device__ double copysign(double a, double b) { return ::copysign(a, b); }
+    static __device__ double frexp(double x, int* e) { return ::frexp(x, e); }
+    static __device__ double ldexp(double x, int e) { return ::ldexp(x, e); }
+};
+template <>
+struct M<float> {
+    static __device__ float exp(float x) { return __expf(x); }
+    static __device__ float log(float x) { return __logf(x); }
+    static __device__ float log1p(float x) { return ::log1pf(x); }
+    static __device__ float atan2(float y, float x) { return ::atan2f(y, x); }
+    static __device__ void sincos(float x, float* s, float* c) { ::sincosf(x, s, c); }
+    static __device__ float fabs(float x) { return ::fabsf(x); }
+    static __device__ float copysign(float a, float b) { return ::copysignf(a, b); }
+    static __device__ float frexp(float x, int* e) { return ::frexpf(x, e); }
+    static __device__ float ldexp(float x, int e) { return ::ldexpf(x, e); }
+};
+
+// tanh(x + iy) = [sgn(x)(1 - E^2) + i 2E sin 2y] / (1 + E^2 + 2E cos 2y),
+// E = exp(-2|x|): multiply numerator and denominator of
+// (sinh 2x + i sin 2y) / (cosh 2x + cos 2y) by 2E; finite for any x.
+template <typename R>
+__device__ inline cplx<R> ctanh(cplx<R> z) {
+    R E = M<R>::exp(-2 * M<R>::fabs(z.re));
+    R s, c;
+    M<R>::sincos(2 * z.im, &s, &c);
+    R den = 1 + E * E + 2 * E * c;
+    R inv = 1 / den;
+    return {M<R>::copysign(1 - E * E, z.re) * inv, 2 * E * s * inv};
+}
+
+// ln cosh(x + iy) = ln cosh x + ln(cos y + i tanh(x) sin y), with
+// ln cosh x = |x| - ln 2 + log1p(exp(-2|x|)); imaginary part is the
+// principal argument.
+template <typename R>
+__device__ inline cplx<R> clncosh(cplx<R> z) {
+    R a = M<R>::fabs(z.re);
+    R E = M<R>::exp(-2 * a);
+    R th = M<R>::copysign((1 - E) / (1 + E), z.re);
+    R s, c;
+    M<R>::sincos(z.im, &s, &c);
+    s *= th;
+    R re = a - (R)0.69314718055994530942 + M<R>::log1p(E) + (R)0.5 * M<R>::log(s * s + c * c);
+    return {re, M<R>::atan2(s, c)};
+}
+
+// complex exp
+template <typename R>
+__device__ inline cplx<R> cexp(cplx<R> z) {
+    R e = M<R>::exp(z.re), s, c;
+    M<R>::sincos(z.im, &s, &c);
+    return {e * c, e * s};
+}
+
+// scale p to a mantissa with max(|re|,|im|) in [0.5, 1) and an exponent
+template <typename R>
+__device__ inline void renorm(cplx<R>& p, int& e) {
+    R m = fmax(M<R>::fabs(p.re), M<R>::fabs(p.im));
+    if (m == 0 || !(m < (R)INFINITY)) return;
+    int k;
+    M<R>::frexp(m, &k);
+    p.re = M<R>::ldexp(p.re, -k);
+    p.im = M<R>::ldexp(p.im, -k);
+    e += k;
+}
+
+// ------------------------------------------------------------------- RNG
+// Philox-4x32-10 (Salmon et al., SC'11) with the published constants.
+struct u4 {
+    uint32_t x, y, z, w;
+};
+__device__ inline u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                            uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {c0, c1, c2, c3};
+}
+// uniform in (0,1): 53 bits for double, 24 bits for float
+template <typename R>
+__device__ inline R uniform(uint32_t a, uint32_t b);
+template <>
+__device__ inline double uniform<double>(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+}
+template <>
+__device__ inline float uniform<float>(uint32_t a, uint32_t) {
+    return ((float)(a >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+__device__ inline uint32_t bounded(uint32_t r, uint32_t n) {
+    return (uint32_t)(((uint64_t)r * n) >> 32);
+}
+
+// ------------------------------------------------------ group reductions
+// Butterfly (xor) reductions over aligned groups of L lanes: every lane of
+// the group ends with the bit-identical result, so control flow that
+// depends on it stays uniform within the group.
+template <int L, typename T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, L);
+    return v;
+}
+template <int L, typename R>
+__device__ inline cplx<R> group_csum(cplx<R> v) {
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) {
+        v.re += __shfl_xor(v.re, o, L);
+        v.im += __shfl_xor(v.im, o, L);
+    }
+    return v;
+}
+// product of complex numbers over the group with exponent tracking; the
+// result is returned as mantissa m and exponent e (value m * 2^e).  With
+// FMA contraction the lanes of a group may differ in the last bit of the
+// result, so callers use it as data only (never for control flow).
+template <int L, typename R>
+__device__ inline void group_cprod(cplx<R>& m, int& e) {
+    renorm(m, e);
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) {
+        cplx<R> q{__shfl_xor(m.re, o, L), __shfl_xor(m.im, o, L)};
+        int qe = __shfl_xor(e, o, L);
+        m = m * q;
+        e += qe;
+        renorm(m, e);
+    }
+}
+
+}  // namespace rbmgpu

@@ -1,0 +1,146 @@
+// kern_api.hpp -- argument blocks of the gfx950 kernels and the host-side
+// launchers (one instantiation per precision in kern_f32.hip / kern_f64.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dev_common.hpp"
+
+namespace rbmgpu {
+
+struct Dims {
+    int N, H, n_alpha, n_symm, n_vb, nw;
+    int w_lds;  // 1: the W table is staged in LDS too (else read from memory)
+};
+
+// weight tables, all complex R; index k*n_alpha + alpha for the N x n_alpha
+// tables (k = permuted site)
+template <typename R>
+struct Tabs {
+    const cplx<R>* W;
+    const cplx<R>* C;   // cosh(2W)
+    const cplx<R>* S;   // sinh(2W)
+    const cplx<R>* hb;  // hidden bias [n_alpha]
+    const cplx<R>* rb;  // ratio bias per site: v_{f mod n_vb} (rbm_base.cpp:310)
+    const cplx<R>* lb;  // log-psi bias per site: sum_i v_i #{s: perm_s(basis_i) = f}
+    const uint8_t* perm;
+};
+
+// local operators as sparse rows: entries [rowptr[rowbase[t] + loc],
+// rowptr[rowbase[t] + loc + 1]) hold (xor mask, value) of row loc of term t
+struct Terms {
+    int n_terms, n_groups;
+    const int32_t* k;
+    const uint8_t* sites;  // n_terms x 8
+    const int32_t* group;
+    const int32_t* rowbase;
+    const int32_t* rowptr;
+    const uint8_t* mask;
+    const double* val;     // complex
+};
+
+template <typename R>
+struct SampArgs {
+    Dims d;
+    Tabs<R> t;
+    Terms T;
+    int n_chains;
+    int64_t n_samples;
+    int warmup, step, npb;
+    uint64_t thr;  // bond flip if philox word < thr (thr = bond_flips * 2^32)
+    uint32_t epoch, k0, k1;
+    const int32_t* pb;
+    uint64_t* states;
+    double* obs;
+    cplx<R>* Tout;
+    uint32_t* acc;
+    uint32_t* prop;
+};
+
+template <typename R>
+struct EvalArgs {
+    Dims d;
+    Tabs<R> t;
+    Terms T;
+    int64_t n;
+    const uint64_t* states;
+    double* logpsi;  // optional
+    double* obs;     // optional
+    cplx<R>* Tout;   // optional
+    double* ratio;   // optional, with flips
+    int32_t flips[8];
+    int n_flips;
+};
+
+template <typename R>
+struct OArgs {
+    Dims d;
+    int64_t S;
+    int P, P_rbm;
+    const uint64_t* states;
+    const cplx<R>* T;
+    const int32_t* vb_site;  // n_symm x n_vb: perm_s(basis_i)
+    const int32_t* invperm;  // n_symm x N
+    cplx<R>* O;
+};
+
+template <typename R>
+struct MomArgs {
+    int64_t S, chunk;
+    int P, n_groups;
+    const cplx<R>* O;
+    const double* obs;
+    double* part;
+};
+
+struct CgStatus {
+    double rsold, rs;
+    int64_t it, itn, max_iter;
+    double rtol;
+    int done;
+};
+
+template <typename R>
+struct SxArgs {
+    int64_t S, nwaves;
+    int P, n_neural;
+    const cplx<R>* O;
+    const double* x;     // complex P
+    const double* mean;  // complex P
+    const double* var;   // P
+    const double* maxvar;
+    double r1, r2, rd;
+    cplx<R>* part;       // nwaves x P
+    double* y;           // complex P
+    const CgStatus* status;
+};
+
+// supported (lanes per chain, units per lane) shapes
+struct Shape {
+    int L, K;
+};
+constexpr Shape kShapes[] = {{8, 1},  {8, 2},  {8, 4},  {8, 8},  {8, 16}, {16, 2},
+                             {16, 4}, {16, 8}, {16, 16}, {32, 2}, {32, 4}, {32, 8},
+                             {32, 16}, {64, 1}, {64, 2}, {64, 4}, {64, 8}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <typename R>
+size_t lds_bytes(const Dims& d) {
+    return (d.w_lds ? 3 : 2) * (size_t)d.N * d.n_alpha * sizeof(cplx<R>) +
+           (d.n_symm > 1 ? (size_t)d.n_symm * d.N : 0);
+}
+
+template <typename R>
+hipError_t launch_sample(const SampArgs<R>& a, Shape s, int block, hipStream_t st);
+template <typename R>
+hipError_t launch_eval(const EvalArgs<R>& a, Shape s, int block, hipStream_t st);
+template <typename R>
+hipError_t launch_build_O(const OArgs<R>& a, hipStream_t st);
+template <typename R>
+hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st);
+template <typename R>
+hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st);
+// ZM (P <= 64*ZM) supported by launch_sx
+int sx_max_params();
+
+}  // namespace rbmgpu

@@ -1,0 +1,502 @@
+// kern_impl.hpp -- gfx950 kernels of the VMC epoch, templated on the real
+// type R (float / double).  Included by kern_f32.hip and kern_f64.hip, which
+// instantiate the launchers for one precision each.
+//
+// Chain layout.  A Markov chain is owned by an aligned group of L lanes of
+// a 64-wide wavefront; hidden unit h = lane + u*L (u < K) lives in that
+// lane's registers as theta_h and tanh(theta_h).  The spin state (<= 128
+// bits) is replicated in every lane of the group.  All lanes of a group draw
+// the same Philox numbers and reduce with xor butterflies, so the group's
+// control flow is uniform without any broadcast.
+//
+// Ratio identity.  Flipping the spins F changes theta_h by
+// delta_h = sum_{f in F} -2 sigma_f W_{k(f,h)}, and
+//   cosh(theta + delta) / cosh(theta) = cosh(delta) + tanh(theta) sinh(delta).
+// cosh/sinh of a single flip's delta are the weight-only tables
+// C = cosh(2W), S = sinh(2W) (with sinh odd in sigma); several flips are
+// composed with the addition theorems.  A proposal therefore costs a few
+// complex multiply-adds per hidden unit and no transcendental; tanh(theta)
+// is recomputed only for accepted moves.
+#pragma once
+#include "dev_common.hpp"
+#include "kern_api.hpp"
+
+namespace rbmgpu {
+
+__device__ inline int bitof(const uint64_t* st, int i) { return (int)((st[i >> 6] >> (i & 63)) & 1); }
+
+template <typename R>
+struct Lds {
+    const cplx<R>* C;
+    const cplx<R>* S;
+    const cplx<R>* W;
+    const uint8_t* P;
+};
+
+// stage the weight tables and symmetry permutations into LDS
+template <typename R>
+__device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, char* smem) {
+    int nw = d.N * d.n_alpha;
+    cplx<R>* C = reinterpret_cast<cplx<R>*>(smem);
+    cplx<R>* S = C + nw;
+    cplx<R>* W = d.w_lds ? S + nw : const_cast<cplx<R>*>(t.W);
+    uint8_t* P = reinterpret_cast<uint8_t*>(S + (d.w_lds ? 2 : 1) * nw);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        C[i] = t.C[i];
+        S[i] = t.S[i];
+        if (d.w_lds) W[i] = t.W[i];
+    }
+    if (d.n_symm > 1)
+        for (int i = threadIdx.x; i < d.n_symm * d.N; i += blockDim.x) P[i] = t.perm[i];
+    __syncthreads();
+    return {C, S, W, P};
+}
+
+__device__ inline int kidx(const Dims& d, const uint8_t* P, int s, int f) {
+    return d.n_symm == 1 ? f : (int)P[s * d.N + f];
+}
+
+template <typename R, int L, int K>
+struct Units {
+    int a[K], s[K];  // alpha and symmetry index per unit, a < 0: padding
+    cplx<R> th[K], tn[K];
+
+    __device__ void init(const Dims& d, int ln) {
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            int h = ln + u * L;
+            a[u] = h < d.H ? h % d.n_alpha : -1;
+            s[u] = h < d.H ? h / d.n_alpha : 0;
+        }
+    }
+    // theta = b + sum_f sigma_f W_{perm_s(f), alpha} (rbm_base.cpp:161-172,
+    // rbm_symmetry.cpp:48-61), then tanh
+    __device__ void from_state(const Dims& d, const Lds<R>& l, const cplx<R>* hb, const uint64_t* st) {
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            if (a[u] < 0) continue;
+            cplx<R> acc = hb[a[u]];
+            for (int f = 0; f < d.N; f++) {
+                cplx<R> w = l.W[kidx(d, l.P, s[u], f) * d.n_alpha + a[u]];
+                if (bitof(st, f)) acc = acc - w;
+                else acc = acc + w;
+            }
+            th[u] = acc;
+            tn[u] = ctanh(acc);
+        }
+    }
+    // lane-partial product of cosh(theta')/cosh(theta) for one or two flips
+    __device__ void ratio12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1, cplx<R>& p,
+                            int& pe) const {
+        p = {1, 0};
+        pe = 0;
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            if (a[u] < 0) continue;
+            int i0 = kidx(d, l.P, s[u], f0) * d.n_alpha + a[u];
+            cplx<R> ch = l.C[i0];
+            cplx<R> sh = (-s0) * l.S[i0];
+            if (f1 >= 0) {
+                int i1 = kidx(d, l.P, s[u], f1) * d.n_alpha + a[u];
+                cplx<R> c1 = l.C[i1];
+                cplx<R> sn1 = (-s1) * l.S[i1];
+                cplx<R> nch = ch * c1 + sh * sn1;
+                sh = sh * c1 + ch * sn1;
+                ch = nch;
+            }
+            p = p * (ch + tn[u] * sh);
+            if (sizeof(R) == 4 && (u & 3) == 3) renorm(p, pe);
+        }
+    }
+    // generic flip set: sites fl[i] for i < 8 with bit i of mask set
+    __device__ void ratio_mask(const Dims& d, const Lds<R>& l, const int* fl, const R* sg, int mask,
+                               cplx<R>& p, int& pe) const {
+        p = {1, 0};
+        pe = 0;
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            if (a[u] < 0) continue;
+            cplx<R> ch{1, 0}, sh{0, 0};
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (!((mask >> i) & 1)) continue;
+                int ii = kidx(d, l.P, s[u], fl[i]) * d.n_alpha + a[u];
+                cplx<R> c = l.C[ii];
+                cplx<R> sn = (-sg[i]) * l.S[ii];
+                cplx<R> nch = ch * c + sh * sn;
+                sh = sh * c + ch * sn;
+                ch = nch;
+            }
+            p = p * (ch + tn[u] * sh);
+            if (sizeof(R) == 4 && (u & 3) == 3) renorm(p, pe);
+        }
+    }
+    // accepted move: theta -= 2 sigma_f W (rbm_base.cpp:174-192)
+    __device__ void accept12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1) {
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            if (a[u] < 0) continue;
+            cplx<R> w0 = l.W[kidx(d, l.P, s[u], f0) * d.n_alpha + a[u]];
+            th[u] = th[u] - (2 * s0) * w0;
+            if (f1 >= 0) {
+                cplx<R> w1 = l.W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
+                th[u] = th[u] - (2 * s1) * w1;
+            }
+            tn[u] = ctanh(th[u]);
+        }
+    }
+};
+
+// psi(sigma')/psi(sigma) for the sites of a term selected by mask: the
+// visible-bias factor exp(-2 sum sigma_f c_f) (rbm_base.cpp:306-318) times
+// the group product of the per-unit cosh ratios
+template <typename R, int L, int K>
+__device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Tabs<R>& t,
+                                          const Units<R, L, K>& U, const int* fl, const R* sg,
+                                          int mask) {
+    cplx<R> bias{0, 0};
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        if ((mask >> i) & 1) bias = bias + (-2 * sg[i]) * t.rb[fl[i]];
+    cplx<R> p;
+    int pe;
+    U.ratio_mask(d, l, fl, sg, mask, p, pe);
+    group_cprod<L>(p, pe);
+    cplx<R> r = cexp(bias) * p;
+    double sc = ::ldexp(1.0, pe);
+    return {(double)r.re * sc, (double)r.im * sc};
+}
+
+// local observables (local_op.cpp:31-60, summed per group as
+// local_op_chain.cpp:54-60); lane 0 writes out[2*g], out[2*g+1]
+template <typename R, int L, int K>
+__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Tabs<R>& t, const Terms& T,
+                                const Units<R, L, K>& U, const uint64_t* st, int ln, double* out) {
+    cplx<double> acc{0, 0};
+    int g = T.n_terms > 0 ? T.group[0] : 0;
+    for (int ti = 0; ti < T.n_terms; ti++) {
+        int gt = T.group[ti];
+        if (gt != g) {
+            if (ln == 0) {
+                out[2 * g] = acc.re;
+                out[2 * g + 1] = acc.im;
+            }
+            acc = {0, 0};
+            g = gt;
+        }
+        int k = T.k[ti];
+        int fl[8];
+        R sg[8];
+        int loc = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            fl[i] = i < k ? (int)T.sites[ti * 8 + i] : 0;
+            int b = i < k ? bitof(st, fl[i]) : 0;
+            sg[i] = b ? (R)-1 : (R)1;
+            loc |= b << i;
+        }
+        int base = T.rowbase[ti] + loc;
+        int e0 = T.rowptr[base], e1 = T.rowptr[base + 1];
+        for (int e = e0; e < e1; e++) {
+            int mask = T.mask[e];
+            cplx<double> v{T.val[2 * e], T.val[2 * e + 1]};
+            if (mask == 0) {
+                acc = acc + v;
+            } else {
+                acc = acc + v * term_ratio<R, L, K>(d, l, t, U, fl, sg, mask);
+            }
+        }
+    }
+    if (ln == 0 && T.n_terms > 0) {
+        out[2 * g] = acc.re;
+        out[2 * g + 1] = acc.im;
+    }
+}
+
+// ------------------------------------------------------------ sampler --
+// metropolis_sampler.cpp:118-273: one chain per group, random initial
+// state, n_warmup + n*step sweeps of N proposal slots; with probability
+// bond_flips both ends of a random non-z bond are flipped (taking two
+// slots), else one uniform site; accept if u < |psi'/psi|^2.  At kept
+// steps the local observables and tanh(theta) of the sample are written.
+template <typename R, int L, int K>
+__global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
+    extern __shared__ __align__(16) char smem[];
+    const Dims d = a.d;
+    Lds<R> l = stage<R>(d, a.t, smem);
+    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    int chain = gtid / L, ln = gtid % L;
+    if (chain >= a.n_chains) return;
+    int64_t spc = a.n_samples / a.n_chains, res = a.n_samples - spc * a.n_chains;
+    int64_t nc = spc + (chain == 0 ? res : 0);
+    int64_t off = chain == 0 ? 0 : chain * spc + res;
+    int64_t total = nc * a.step + a.warmup;
+
+    uint64_t st[2] = {0, 0};
+    {
+        u4 r = philox((uint32_t)chain, a.epoch, 0xFFFFFFFFu, 0u, a.k0, a.k1);
+        uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) st[q >> 1] |= (uint64_t)w[q] << (32 * (q & 1));
+        if (d.N < 64) st[0] &= (((uint64_t)1) << d.N) - 1;
+        if (d.N <= 64) st[1] = 0;
+        else if (d.N < 128) st[1] &= (((uint64_t)1) << (d.N - 64)) - 1;
+    }
+    Units<R, L, K> U;
+    U.init(d, ln);
+    U.from_state(d, l, a.t.hb, st);
+
+    uint32_t acc = 0, prop = 0;
+    int64_t k = 0;
+    for (int64_t step = 0; step < total; step++) {
+        for (int sw = 0; sw < d.N; sw++) {
+            u4 r = philox((uint32_t)chain, a.epoch, (uint32_t)step, (uint32_t)sw, a.k0, a.k1);
+            int f0, f1 = -1;
+            if ((uint64_t)r.x < a.thr && a.npb > 0) {
+                int bi = (int)bounded(r.y, (uint32_t)a.npb);
+                f0 = a.pb[2 * bi];
+                f1 = a.pb[2 * bi + 1];
+                sw++;
+            } else {
+                f0 = (int)bounded(r.y, (uint32_t)d.N);
+            }
+            prop++;
+            R s0 = bitof(st, f0) ? (R)-1 : (R)1;
+            R s1 = (f1 >= 0 && bitof(st, f1)) ? (R)-1 : (R)1;
+            cplx<R> p;
+            int pe;
+            U.ratio12(d, l, f0, s0, f1, s1, p, pe);
+            R lp = M<R>::log(norm2(p)) + (R)(2 * pe) * (R)0.69314718055994530942;
+            R lnr2 = group_sum<L>(lp);
+            R bre = (-2 * s0) * a.t.rb[f0].re;
+            if (f1 >= 0) bre += (-2 * s1) * a.t.rb[f1].re;
+            lnr2 += 2 * bre;
+            R uu = uniform<R>(r.z, r.w);
+            if (uu < M<R>::exp(lnr2)) {
+                U.accept12(d, l, f0, s0, f1, s1);
+                st[f0 >> 6] ^= (uint64_t)1 << (f0 & 63);
+                if (f1 >= 0) st[f1 >> 6] ^= (uint64_t)1 << (f1 & 63);
+                acc++;
+            }
+        }
+        if (step >= a.warmup && (step - a.warmup) % a.step == 0) {
+            int64_t smp = off + k;
+            if (ln == 0)
+                for (int w = 0; w < d.nw; w++) a.states[smp * d.nw + w] = st[w];
+#pragma unroll
+            for (int u = 0; u < K; u++)
+                if (U.a[u] >= 0) a.Tout[smp * d.H + ln + u * L] = U.tn[u];
+            eval_obs<R, L, K>(d, l, a.t, a.T, U, st, ln, a.obs + smp * 2 * a.T.n_groups);
+            k++;
+        }
+    }
+    if (ln == 0) {
+        a.acc[chain] = acc;
+        a.prop[chain] = prop;
+    }
+}
+
+// ---------------------------------------------------------- eval states --
+// one group per host-given state: theta from scratch, tanh, optionally
+// log psi, local observables and the psi ratio for given flips
+template <typename R, int L, int K>
+__global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
+    extern __shared__ __align__(16) char smem[];
+    const Dims d = a.d;
+    Lds<R> l = stage<R>(d, a.t, smem);
+    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = gtid / L;
+    int ln = gtid % L;
+    if (g >= a.n) return;
+    uint64_t st[2] = {0, 0};
+    for (int w = 0; w < d.nw; w++) st[w] = a.states[g * d.nw + w];
+    Units<R, L, K> U;
+    U.init(d, ln);
+    U.from_state(d, l, a.t.hb, st);
+    if (a.Tout) {
+#pragma unroll
+        for (int u = 0; u < K; u++)
+            if (U.a[u] >= 0) a.Tout[g * d.H + ln + u * L] = U.tn[u];
+    }
+    if (a.logpsi) {
+        // log psi = sum_f c_f sigma_f + sum_h ln cosh theta_h (rbm_base.cpp:294-304)
+        cplx<R> s{0, 0};
+#pragma unroll
+        for (int u = 0; u < K; u++)
+            if (U.a[u] >= 0) s = s + clncosh(U.th[u]);
+        for (int f = ln; f < d.N; f += L) {
+            R sg = bitof(st, f) ? (R)-1 : (R)1;
+            s = s + sg * a.t.lb[f];
+        }
+        s = group_csum<L>(s);
+        if (ln == 0) {
+            a.logpsi[2 * g] = s.re;
+            a.logpsi[2 * g + 1] = s.im;
+        }
+    }
+    if (a.obs) eval_obs<R, L, K>(d, l, a.t, a.T, U, st, ln, a.obs + g * 2 * a.T.n_groups);
+    if (a.ratio) {
+        int fl[8];
+        R sg[8];
+        int mask = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            fl[i] = i < a.n_flips ? a.flips[i] : 0;
+            sg[i] = bitof(st, fl[i]) ? (R)-1 : (R)1;
+            if (i < a.n_flips) mask |= 1 << i;
+        }
+        cplx<double> r = term_ratio<R, L, K>(d, l, a.t, U, fl, sg, mask);
+        if (ln == 0) {
+            a.ratio[2 * g] = r.re;
+            a.ratio[2 * g + 1] = r.im;
+        }
+    }
+}
+
+// ----------------------------------------------------------- O matrix --
+// O column of one sample (rbm_base.cpp:194-217, rbm_symmetry.cpp:97-130):
+// [ sum_s sigma_{perm_s(basis_i)} | sum_s tanh theta(a,s) |
+//   sum_s (P_s sigma)_j tanh theta(a,s) at j + a*N ]
+template <typename R>
+__global__ void __launch_bounds__(256) k_build_O(OArgs<R> a) {
+    extern __shared__ __align__(16) char smem[];
+    const Dims d = a.d;
+    float* sg = reinterpret_cast<float*>(smem);
+    cplx<R>* tr = reinterpret_cast<cplx<R>*>(sg + 128);
+    int64_t k = blockIdx.x;
+    if (k >= a.S) return;
+    for (int f = threadIdx.x; f < d.N; f += blockDim.x)
+        sg[f] = ((a.states[k * d.nw + (f >> 6)] >> (f & 63)) & 1) ? -1.f : 1.f;
+    for (int h = threadIdx.x; h < d.H; h += blockDim.x) tr[h] = a.T[k * d.H + h];
+    __syncthreads();
+    cplx<R>* o = a.O + k * a.P;
+    int nvb = d.n_vb, na = d.n_alpha;
+    for (int p = threadIdx.x; p < a.P_rbm; p += blockDim.x) {
+        cplx<R> v{0, 0};
+        if (p < nvb) {
+            R acc = 0;
+            for (int s = 0; s < d.n_symm; s++) acc += (R)sg[a.vb_site[s * nvb + p]];
+            v = {acc, 0};
+        } else if (p < nvb + na) {
+            int al = p - nvb;
+            for (int s = 0; s < d.n_symm; s++) v = v + tr[al + s * na];
+        } else {
+            int q = p - nvb - na, j = q % d.N, al = q / d.N;
+            if (d.n_symm == 1) {
+                v = (R)sg[j] * tr[al];
+            } else {
+                for (int s = 0; s < d.n_symm; s++) v = v + (R)sg[a.invperm[s * d.N + j]] * tr[al + s * na];
+            }
+        }
+        o[p] = v;
+    }
+}
+
+// ------------------------------------------------------------ moments --
+// per-block partial sums over a chunk of samples: sum O, sum |O|^2,
+// sum E conj(O) (accumulated in double)
+template <typename R>
+__global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
+    int64_t k0 = (int64_t)blockIdx.x * a.chunk, k1 = k0 + a.chunk;
+    if (k1 > a.S) k1 = a.S;
+    for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
+        double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0;
+        for (int64_t k = k0; k < k1; k++) {
+            cplx<R> o = a.O[k * a.P + p];
+            double ore = o.re, oim = o.im;
+            double ere = a.obs[k * 2 * a.n_groups], eim = a.obs[k * 2 * a.n_groups + 1];
+            s0 += ore;
+            s1 += oim;
+            s2 += ore * ore + oim * oim;
+            e0 += ere * ore + eim * oim;  // E * conj(O)
+            e1 += eim * ore - ere * oim;
+        }
+        double* q = a.part + ((int64_t)blockIdx.x * a.P + p) * 5;
+        q[0] = s0;
+        q[1] = s1;
+        q[2] = s2;
+        q[3] = e0;
+        q[4] = e1;
+    }
+}
+
+// ------------------------------------------------------------------ S.x --
+// one pass over O: every wavefront streams whole rows k (samples), forms
+// y_k = O_k . b with a wave reduction and accumulates conj(O_k) y_k into
+// lane-strided registers; per-wave partials are summed by k_sx_finish
+template <typename R, int ZM>
+__global__ void __launch_bounds__(256) k_sx_partial(SxArgs<R> a) {
+    extern __shared__ __align__(16) char smem[];
+    if (a.status && a.status->done) return;
+    cplx<R>* b = reinterpret_cast<cplx<R>*>(smem);
+    for (int p = threadIdx.x; p < a.P; p += blockDim.x) b[p] = {(R)a.x[2 * p], (R)a.x[2 * p + 1]};
+    __syncthreads();
+    int lane = threadIdx.x & 63;
+    int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    cplx<R> z[ZM];
+#pragma unroll
+    for (int i = 0; i < ZM; i++) z[i] = {0, 0};
+    for (int64_t k = wave; k < a.S; k += nwaves) {
+        const cplx<R>* o = a.O + k * a.P;
+        cplx<R> dot{0, 0};
+#pragma unroll
+        for (int i = 0; i < ZM; i++) {
+            int p = lane + 64 * i;
+            if (p < a.P) dot = dot + o[p] * b[p];
+        }
+        dot = group_csum<64>(dot);
+#pragma unroll
+        for (int i = 0; i < ZM; i++) {
+            int p = lane + 64 * i;
+            if (p < a.P) z[i] = z[i] + conj(o[p]) * dot;
+        }
+    }
+    cplx<R>* out = a.part + wave * a.P;
+#pragma unroll
+    for (int i = 0; i < ZM; i++) {
+        int p = lane + 64 * i;
+        if (p < a.P) out[p] = z[i];
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_sx_finish(SxArgs<R> a) {
+    if (a.status && a.status->done) return;
+    __shared__ double red[2][256];
+    // mu^T x (computed by every block; P is small)
+    double mr = 0, mi = 0;
+    for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
+        double ur = a.mean[2 * p], ui = a.mean[2 * p + 1], xr = a.x[2 * p], xi = a.x[2 * p + 1];
+        mr += ur * xr - ui * xi;
+        mi += ur * xi + ui * xr;
+    }
+    red[0][threadIdx.x] = mr;
+    red[1][threadIdx.x] = mi;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    mr = red[0][0];
+    mi = red[1][0];
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    double zr = 0, zi = 0;
+    for (int64_t w = 0; w < a.nwaves; w++) {
+        cplx<R> v = a.part[w * a.P + p];
+        zr += v.re;
+        zi += v.im;
+    }
+    double inv = 1.0 / (double)a.S;
+    double reg = a.var[p] * (p < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
+    double xr = a.x[2 * p], xi = a.x[2 * p + 1];
+    double ur = a.mean[2 * p], ui = -a.mean[2 * p + 1];  // conj(mu_p)
+    a.y[2 * p] = zr * inv + reg * xr - (ur * mr - ui * mi);
+    a.y[2 * p + 1] = zi * inv + reg * xi - (ur * mi + ui * mr);
+}
+
+}  // namespace rbmgpu

@@ -1,0 +1,96 @@
+// kern_launch.hpp -- launchers that dispatch the runtime launch shape to the
+// compiled template instances; included once per precision.
+#pragma once
+#include "kern_impl.hpp"
+
+namespace rbmgpu {
+
+template <typename R, int L, int K>
+static hipError_t sample_one(const SampArgs<R>& a, int block, hipStream_t st) {
+    int64_t threads = (int64_t)a.n_chains * L;
+    int grid = (int)((threads + block - 1) / block);
+    size_t lds = lds_bytes<R>(a.d);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sample<R, L, K>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sample<R, L, K>), dim3(grid), dim3(block), lds, st, a);
+    return hipGetLastError();
+}
+template <typename R, int L, int K>
+static hipError_t eval_one(const EvalArgs<R>& a, int block, hipStream_t st) {
+    int64_t threads = a.n * L;
+    int grid = (int)((threads + block - 1) / block);
+    size_t lds = lds_bytes<R>(a.d);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_eval<R, L, K>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_eval<R, L, K>), dim3(grid), dim3(block), lds, st, a);
+    return hipGetLastError();
+}
+
+#define RBM_SHAPES(X) \
+    X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(8, 16) X(16, 2) X(16, 4) X(16, 8) X(16, 16) X(32, 2) \
+    X(32, 4) X(32, 8) X(32, 16) X(64, 1) X(64, 2) X(64, 4) X(64, 8)
+
+template <typename R>
+hipError_t launch_sample(const SampArgs<R>& a, Shape s, int block, hipStream_t st) {
+#define X(L_, K_) \
+    if (s.L == L_ && s.K == K_) return sample_one<R, L_, K_>(a, block, st);
+    RBM_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+template <typename R>
+hipError_t launch_eval(const EvalArgs<R>& a, Shape s, int block, hipStream_t st) {
+#define X(L_, K_) \
+    if (s.L == L_ && s.K == K_) return eval_one<R, L_, K_>(a, block, st);
+    RBM_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_build_O(const OArgs<R>& a, hipStream_t st) {
+    size_t lds = 128 * sizeof(float) + (size_t)a.d.H * sizeof(cplx<R>);
+    hipLaunchKernelGGL((k_build_O<R>), dim3((unsigned)a.S), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_o_partial<R>), dim3(blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename R, int ZM>
+static hipError_t sx_one(const SxArgs<R>& a, int blocks, hipStream_t st) {
+    size_t lds = (size_t)a.P * sizeof(cplx<R>);
+    hipLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), lds, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_sx_finish<R>), dim3((a.P + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st) {
+    if (a.P <= 64 * 2) return sx_one<R, 2>(a, blocks, st);
+    if (a.P <= 64 * 4) return sx_one<R, 4>(a, blocks, st);
+    if (a.P <= 64 * 8) return sx_one<R, 8>(a, blocks, st);
+    if (a.P <= 64 * 16) return sx_one<R, 16>(a, blocks, st);
+    if (a.P <= 64 * 24) return sx_one<R, 24>(a, blocks, st);
+    if (a.P <= 64 * 36) return sx_one<R, 36>(a, blocks, st);
+    if (a.P <= 64 * 48) return sx_one<R, 48>(a, blocks, st);
+    return hipErrorInvalidValue;
+}
+
+#define RBM_INSTANTIATE(R)                                                                  \
+    template hipError_t launch_sample<R>(const SampArgs<R>&, Shape, int, hipStream_t);    \
+    template hipError_t launch_eval<R>(const EvalArgs<R>&, Shape, int, hipStream_t);      \
+    template hipError_t launch_build_O<R>(const OArgs<R>&, hipStream_t);                  \
+    template hipError_t launch_o_partial<R>(const MomArgs<R>&, int, hipStream_t);         \
+    template hipError_t launch_sx<R>(const SxArgs<R>&, int, hipStream_t);
+
+}  // namespace rbmgpu

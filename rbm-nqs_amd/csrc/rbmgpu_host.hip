@@ -1,0 +1,1012 @@
+// rbmgpu_host.hip -- context, device buffers and the C ABI of include/rbmgpu.h.
+//
+// One context holds the model tables, the weight tables of the current
+// parameters, the sample buffers of one epoch and the CG workspace, all in
+// HBM on the current device, and one HIP stream (default priority).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rbmgpu.h"
+#include "kern_api.hpp"
+
+using namespace rbmgpu;
+using zd = std::complex<double>;
+
+int rbmgpu::sx_max_params() { return 64 * 48; }
+
+struct rbmgpu_ctx {
+    // ---- model
+    int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0;
+    int64_t P = 0, P_rbm = 0;
+    std::vector<int32_t> perm, basis, pb;
+    int n_terms = 0, n_groups = 1;
+    // ---- sampler
+    int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
+    int64_t n_samples = 0;
+    double bond_flips = 0.5;
+    uint64_t seed = 0;
+    std::vector<zd> params;
+    // ---- device
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    size_t lds_limit = 65536;
+    void *d_W = nullptr, *d_C = nullptr, *d_S = nullptr, *d_hb = nullptr, *d_rb = nullptr,
+         *d_lb = nullptr;
+    uint8_t* d_perm = nullptr;
+    int32_t *d_invperm = nullptr, *d_vbsite = nullptr, *d_pb = nullptr;
+    int32_t *d_tk = nullptr, *d_tgroup = nullptr, *d_rowbase = nullptr, *d_rowptr = nullptr;
+    uint8_t *d_tsites = nullptr, *d_mask = nullptr;
+    double* d_val = nullptr;
+    // samples
+    int64_t cap = 0, S = 0;
+    uint64_t* d_states = nullptr;
+    double* d_obs = nullptr;
+    void* d_T = nullptr;
+    void* d_O = nullptr;
+    uint32_t *d_acc = nullptr, *d_prop = nullptr;
+    bool have_chains = false;
+    // moments
+    double *d_part = nullptr, *d_omean = nullptr, *d_ovar = nullptr, *d_eo = nullptr,
+           *d_maxvar = nullptr, *d_emom = nullptr;
+    int64_t part_cap = 0;
+    bool moments_valid = false;
+    // CG / S.x
+    double *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr;
+    CgStatus* d_st = nullptr;
+    void* d_sxpart = nullptr;
+    int64_t sx_waves = 0, sxpart_cap = 0;
+    // eval scratch
+    uint64_t* d_evst = nullptr;
+    double *d_evout = nullptr, *d_evobs = nullptr;
+    int64_t ev_cap = 0;
+    // launch shape
+    Shape shape{0, 0};
+    int block = 256;
+    // timing
+    hipEvent_t ev[12] = {};
+    double times[5] = {0, 0, 0, 0, 0};
+    std::string err;
+};
+
+namespace {
+
+#define HIPCK(x)                                                                   \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);             \
+            return -2;                                                             \
+        }                                                                          \
+    } while (0)
+
+int fail(rbmgpu_ctx* ctx, const std::string& m) {
+    ctx->err = m;
+    return -1;
+}
+
+template <typename T>
+int dalloc(rbmgpu_ctx* ctx, T** p, size_t bytes) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (bytes == 0) bytes = 16;
+    HIPCK(hipMalloc((void**)p, bytes));
+    return 0;
+}
+template <typename T>
+int upload(rbmgpu_ctx* ctx, T** p, const void* src, size_t bytes) {
+    if (int rc = dalloc(ctx, p, bytes)) return rc;
+    if (bytes) HIPCK(hipMemcpy(*p, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+size_t rsize(const rbmgpu_ctx* c) { return c->prec == RBMGPU_FP32 ? 4 : 8; }
+
+Dims dims(const rbmgpu_ctx* c) {
+    Dims d{c->N, c->H, c->na, c->nsymm, c->nvb, c->nw, 1};
+    size_t need = (c->prec == RBMGPU_FP32 ? lds_bytes<float>(d) : lds_bytes<double>(d));
+    if (need > c->lds_limit) d.w_lds = 0;
+    return d;
+}
+
+// weight tables from the FP64 parameter vector (rbm_base.cpp:117-131 layout)
+int build_weight_tables(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    int N = c->N, na = c->na, nvb = c->nvb;
+    std::vector<zd> W(N * na), C(N * na), S(N * na), hb(na), rb(N), lb(N, 0.);
+    for (int k = 0; k < N; k++)
+        for (int a = 0; a < na; a++) {
+            zd w = c->params[nvb + na + k + (int64_t)a * N];
+            W[k * na + a] = w;
+            C[k * na + a] = std::cosh(2. * w);
+            S[k * na + a] = std::sinh(2. * w);
+        }
+    for (int a = 0; a < na; a++) hb[a] = c->params[nvb + a];
+    for (int f = 0; f < N; f++) rb[f] = c->params[f % nvb];
+    for (int i = 0; i < nvb; i++)
+        for (int s = 0; s < c->nsymm; s++) lb[c->perm[s * N + c->basis[i]]] += c->params[i];
+    auto put = [&](void** dst, const std::vector<zd>& v) -> int {
+        if (c->prec == RBMGPU_FP64) return upload(ctx, dst, v.data(), v.size() * sizeof(zd));
+        std::vector<std::complex<float>> f(v.begin(), v.end());
+        return upload(ctx, dst, f.data(), f.size() * sizeof(std::complex<float>));
+    };
+    if (int rc = put(&c->d_W, W)) return rc;
+    if (int rc = put(&c->d_C, C)) return rc;
+    if (int rc = put(&c->d_S, S)) return rc;
+    if (int rc = put(&c->d_hb, hb)) return rc;
+    if (int rc = put(&c->d_rb, rb)) return rc;
+    if (int rc = put(&c->d_lb, lb)) return rc;
+    c->moments_valid = false;
+    return 0;
+}
+
+template <typename R>
+Tabs<R> tabs(const rbmgpu_ctx* c) {
+    return {(const cplx<R>*)c->d_W,  (const cplx<R>*)c->d_C,  (const cplx<R>*)c->d_S,
+            (const cplx<R>*)c->d_hb, (const cplx<R>*)c->d_rb, (const cplx<R>*)c->d_lb,
+            c->d_perm};
+}
+Terms terms(const rbmgpu_ctx* c) {
+    return {c->n_terms, c->n_groups, c->d_tk,     c->d_tsites, c->d_tgroup,
+            c->d_rowbase, c->d_rowptr, c->d_mask, c->d_val};
+}
+
+int ensure_samples(rbmgpu_ctx* c, int64_t S) {
+    rbmgpu_ctx* ctx = c;
+    if (S <= c->cap) return 0;
+    if (int rc = dalloc(ctx, &c->d_states, S * c->nw * sizeof(uint64_t))) return rc;
+    if (int rc = dalloc(ctx, &c->d_obs, S * 2 * c->n_groups * sizeof(double))) return rc;
+    if (int rc = dalloc(ctx, &c->d_T, S * c->H * 2 * rsize(c))) return rc;
+    if (c->keep_o)
+        if (int rc = dalloc(ctx, &c->d_O, S * c->P * 2 * rsize(c))) return rc;
+    c->cap = S;
+    return 0;
+}
+
+bool shape_fits(const rbmgpu_ctx* c, Shape s) { return s.L * s.K >= c->H; }
+
+// block size: 256 threads unless that leaves fewer than two blocks per CU
+int pick_block(const rbmgpu_ctx* c, int L, int64_t groups) {
+    int b = 256;
+    while (b > 64 && groups * L / b < 2 * c->n_cu) b /= 2;
+    return std::max(b, L);
+}
+
+template <typename R>
+int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
+    rbmgpu_ctx* ctx = c;
+    SampArgs<R> a{};
+    a.d = dims(c);
+    a.t = tabs<R>(c);
+    a.T = terms(c);
+    a.n_chains = c->n_chains;
+    a.n_samples = n_samples;
+    a.warmup = warmup;
+    a.step = c->step;
+    a.npb = (int)c->pb.size() / 2;
+    double thr = std::floor(c->bond_flips * 4294967296.0);
+    a.thr = (uint64_t)std::min(std::max(thr, 0.0), 4294967296.0);
+    a.epoch = epoch;
+    a.k0 = (uint32_t)c->seed;
+    a.k1 = (uint32_t)(c->seed >> 32);
+    a.pb = c->d_pb;
+    a.states = c->d_states;
+    a.obs = c->d_obs;
+    a.Tout = (cplx<R>*)c->d_T;
+    a.acc = c->d_acc;
+    a.prop = c->d_prop;
+    HIPCK(launch_sample<R>(a, s, block, c->stream));
+    return 0;
+}
+
+template <typename R>
+int run_build_O(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (!c->keep_o || c->S == 0) return 0;
+    OArgs<R> a{};
+    a.d = dims(c);
+    a.S = c->S;
+    a.P = (int)c->P;
+    a.P_rbm = (int)c->P_rbm;
+    a.states = c->d_states;
+    a.T = (const cplx<R>*)c->d_T;
+    a.vb_site = c->d_vbsite;
+    a.invperm = c->d_invperm;
+    a.O = (cplx<R>*)c->d_O;
+    HIPCK(launch_build_O<R>(a, c->stream));
+    return 0;
+}
+
+int autotune(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (c->shape.L) return 0;
+    std::vector<Shape> cand;
+    for (int L : {8, 16, 32, 64}) {
+        Shape best{0, 0};
+        for (int i = 0; i < kNumShapes; i++) {
+            Shape s = kShapes[i];
+            if (s.L == L && shape_fits(c, s) && (best.L == 0 || s.K < best.K)) best = s;
+        }
+        if (best.L && (c->lanes_req == 0 || c->lanes_req == L)) cand.push_back(best);
+    }
+    if (cand.empty()) return fail(c, "no launch shape for this number of hidden units");
+    if (cand.size() == 1) {
+        c->shape = cand[0];
+        c->block = pick_block(c, c->shape.L, c->n_chains);
+        return 0;
+    }
+    // time a short epoch (1 sample per chain, 2 warmup sweeps) per shape
+    if (int rc = ensure_samples(c, c->n_chains)) return rc;
+    float best_ms = 1e30f;
+    for (Shape s : cand) {
+        int block = pick_block(c, s.L, c->n_chains);
+        float ms = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            HIPCK(hipEventRecord(c->ev[10], c->stream));
+            int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, 0xFFFFFFF0u, s, block, c->n_chains, 2)
+                                            : run_sample<double>(c, 0xFFFFFFF0u, s, block, c->n_chains, 2);
+            if (rc) return rc;
+            HIPCK(hipEventRecord(c->ev[11], c->stream));
+            HIPCK(hipEventSynchronize(c->ev[11]));
+            HIPCK(hipEventElapsedTime(&ms, c->ev[10], c->ev[11]));
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            c->shape = s;
+            c->block = block;
+        }
+    }
+    return 0;
+}
+
+float elapsed(rbmgpu_ctx* c, int a, int b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0;
+    return ms;
+}
+
+// ---- reductions in FP64 (aggregator.cpp:84-239, one rank, unit weights)
+__global__ void __launch_bounds__(1024) k_e_moments(const double* obs, int64_t S, int n_groups, int64_t n_bins,
+                                                    double* out) {
+    __shared__ double ra[1024], rb[1024];
+    auto reduce = [&](double& x, double& y) {
+        ra[threadIdx.x] = x;
+        rb[threadIdx.x] = y;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                ra[threadIdx.x] += ra[threadIdx.x + s];
+                rb[threadIdx.x] += rb[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        x = ra[0];
+        y = rb[0];
+        __syncthreads();
+    };
+    int G = 2 * n_groups;
+    // means of every group
+    for (int g = 0; g < n_groups; g++) {
+        double sr = 0, si = 0;
+        for (int64_t k = threadIdx.x; k < S; k += blockDim.x) {
+            sr += obs[k * G + 2 * g];
+            si += obs[k * G + 2 * g + 1];
+        }
+        reduce(sr, si);
+        if (threadIdx.x == 0) {
+            out[16 + 2 * g] = sr / (double)S;
+            out[16 + 2 * g + 1] = si / (double)S;
+        }
+    }
+    __syncthreads();
+    double mr = out[16], mi = out[17];
+    double v = 0, dummy = 0;
+    for (int64_t k = threadIdx.x; k < S; k += blockDim.x) {
+        double dr = obs[k * G] - mr, di = obs[k * G + 1] - mi;
+        v += dr * dr + di * di;
+    }
+    reduce(v, dummy);
+    double vb = 0;
+    int64_t bs = n_bins > 1 ? S / n_bins : 0;
+    if (n_bins > 1) {
+        for (int64_t b = 0; b < n_bins; b++) {
+            double br = 0, bi = 0;
+            for (int64_t k = b * bs + threadIdx.x; k < (b + 1) * bs; k += blockDim.x) {
+                br += obs[k * G];
+                bi += obs[k * G + 1];
+            }
+            reduce(br, bi);
+            double dr = br / bs - mr, di = bi / bs - mi;
+            vb += bs * (dr * dr + di * di);
+        }
+    }
+    if (threadIdx.x == 0) {
+        double denom = (double)S - 1.0;  // ptotal - wsum2 / ptotal
+        double tau = 0;
+        if (n_bins > 1 && !(v == 0 && vb == 0)) tau = 0.5 * (fmax(bs * vb / v, 1.0) - 1);
+        double var = fmax(v / denom, 0.0), varb = fmax(vb / denom, 0.0);
+        out[0] = mr;
+        out[1] = mi;
+        out[2] = var;
+        out[3] = varb;
+        out[4] = tau;
+        out[5] = n_bins > 1 ? sqrt(varb / (double)n_bins) : sqrt(var / (double)S);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb, int64_t S, int P,
+                                                 double* mean, double* var, double* eo,
+                                                 unsigned long long* maxvar) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double q[5] = {0, 0, 0, 0, 0};
+    for (int64_t b = 0; b < nb; b++)
+        for (int c = 0; c < 5; c++) q[c] += part[(b * P + p) * 5 + c];
+    double inv = 1.0 / (double)S;
+    double mr = q[0] * inv, mi = q[1] * inv;
+    mean[2 * p] = mr;
+    mean[2 * p + 1] = mi;
+    double v = (q[2] - (double)S * (mr * mr + mi * mi)) / ((double)S - 1.0);
+    v = fmax(v, 0.0);
+    var[p] = v;
+    eo[2 * p] = q[3] * inv;
+    eo[2 * p + 1] = q[4] * inv;
+    atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---- CG on the device, FP64 vectors (cg_solver.cpp:119-189)
+__device__ void block_sum2(double& x, double& y) {
+    __shared__ double ra[1024], rb[1024];
+    ra[threadIdx.x] = x;
+    rb[threadIdx.x] = y;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ra[threadIdx.x] += ra[threadIdx.x + s];
+            rb[threadIdx.x] += rb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    x = ra[0];
+    y = rb[0];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double* Ax, double* r, double* p,
+                                                  int P, CgStatus* st, int64_t max_iter, double rtol) {
+    double s = 0, dummy = 0;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        double rr = b[2 * i] - Ax[2 * i], ri = b[2 * i + 1] - Ax[2 * i + 1];
+        r[2 * i] = rr;
+        r[2 * i + 1] = ri;
+        p[2 * i] = rr;
+        p[2 * i + 1] = ri;
+        s += rr * rr + ri * ri;
+    }
+    block_sum2(s, dummy);
+    if (threadIdx.x == 0) {
+        st->rsold = s;
+        st->rs = s;
+        st->it = 0;
+        st->itn = max_iter;
+        st->max_iter = max_iter;
+        st->rtol = rtol;
+        st->done = max_iter <= 0;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_cg_step(double* x, double* r, double* p, const double* Ap, int P,
+                                                  CgStatus* st) {
+    if (st->done) return;
+    double ar = 0, ai = 0;  // p^H Ap
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        double pr = p[2 * i], pi = p[2 * i + 1], qr = Ap[2 * i], qi = Ap[2 * i + 1];
+        ar += pr * qr + pi * qi;
+        ai += pr * qi - pi * qr;
+    }
+    block_sum2(ar, ai);
+    double rsold = st->rsold;
+    double den = ar * ar + ai * ai;
+    double alr = rsold * ar / den, ali = -rsold * ai / den;  // rsold / (ar + i ai)
+    double s = 0, dummy = 0;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        double pr = p[2 * i], pi = p[2 * i + 1], qr = Ap[2 * i], qi = Ap[2 * i + 1];
+        x[2 * i] += alr * pr - ali * pi;
+        x[2 * i + 1] += alr * pi + ali * pr;
+        double rr = r[2 * i] - (alr * qr - ali * qi), ri = r[2 * i + 1] - (alr * qi + ali * qr);
+        r[2 * i] = rr;
+        r[2 * i + 1] = ri;
+        s += rr * rr + ri * ri;
+    }
+    block_sum2(s, dummy);
+    bool conv = s < st->rtol;
+    if (!conv) {
+        double beta = s / rsold;
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            p[2 * i] = r[2 * i] + beta * p[2 * i];
+            p[2 * i + 1] = r[2 * i + 1] + beta * p[2 * i + 1];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->rs = s;
+        if (conv) {
+            st->itn = st->it;
+            st->done = 1;
+        } else {
+            st->rsold = s;
+            st->it += 1;
+            if (st->it >= st->max_iter) {
+                st->itn = st->max_iter;
+                st->done = 1;
+            }
+        }
+    }
+}
+
+int ensure_cg(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (c->d_x) return 0;
+    size_t vb = c->P * 2 * sizeof(double);
+    if (int rc = dalloc(ctx, &c->d_x, vb)) return rc;
+    if (int rc = dalloc(ctx, &c->d_b, vb)) return rc;
+    if (int rc = dalloc(ctx, &c->d_r, vb)) return rc;
+    if (int rc = dalloc(ctx, &c->d_p, vb)) return rc;
+    if (int rc = dalloc(ctx, &c->d_Ap, vb)) return rc;
+    if (int rc = dalloc(ctx, &c->d_st, sizeof(CgStatus))) return rc;
+    return 0;
+}
+
+int ensure_sx(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (c->P > sx_max_params()) return fail(c, "too many parameters for the S.x kernel");
+    int64_t w = std::min<int64_t>(std::max<int64_t>(c->S / 32, 4), 4096);
+    w = (w + 3) / 4 * 4;
+    c->sx_waves = w;
+    int64_t need = w * c->P * 2 * rsize(c);
+    if (need > c->sxpart_cap) {
+        if (int rc = dalloc(ctx, &c->d_sxpart, need)) return rc;
+        c->sxpart_cap = need;
+    }
+    return 0;
+}
+
+template <typename R>
+int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st) {
+    rbmgpu_ctx* ctx = c;
+    SxArgs<R> a{};
+    a.S = c->S;
+    a.nwaves = c->sx_waves;
+    a.P = (int)c->P;
+    a.n_neural = (int)c->P_rbm;
+    a.O = (const cplx<R>*)c->d_O;
+    a.x = x;
+    a.mean = c->d_omean;
+    a.var = c->d_ovar;
+    a.maxvar = c->d_maxvar;
+    a.r1 = r1;
+    a.r2 = r2;
+    a.rd = rd;
+    a.part = (cplx<R>*)c->d_sxpart;
+    a.y = y;
+    a.status = st;
+    HIPCK(launch_sx<R>(a, (int)(c->sx_waves / 4), c->stream));
+    return 0;
+}
+
+int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st);
+}
+
+int compute_moments(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (c->S == 0) return fail(c, "no samples");
+    if (!c->d_emom) {
+        if (int rc = dalloc(ctx, &c->d_emom, (16 + 2 * c->n_groups) * sizeof(double))) return rc;
+        if (int rc = dalloc(ctx, &c->d_omean, c->P * 2 * sizeof(double))) return rc;
+        if (int rc = dalloc(ctx, &c->d_ovar, c->P * sizeof(double))) return rc;
+        if (int rc = dalloc(ctx, &c->d_eo, c->P * 2 * sizeof(double))) return rc;
+        if (int rc = dalloc(ctx, &c->d_maxvar, sizeof(double))) return rc;
+    }
+    int64_t nb = 50;
+    while (c->S % nb != 0) nb--;
+    HIPCK(hipEventRecord(c->ev[4], c->stream));
+    hipLaunchKernelGGL(k_e_moments, dim3(1), dim3(1024), 0, c->stream, c->d_obs, c->S, c->n_groups, nb,
+                       c->d_emom);
+    HIPCK(hipGetLastError());
+    if (c->keep_o) {
+        int64_t chunk = 256;
+        int64_t blocks = (c->S + chunk - 1) / chunk;
+        if (blocks > 4096) {
+            blocks = 4096;
+            chunk = (c->S + blocks - 1) / blocks;
+            blocks = (c->S + chunk - 1) / chunk;
+        }
+        int64_t need = blocks * c->P * 5;
+        if (need > c->part_cap) {
+            if (int rc = dalloc(ctx, &c->d_part, need * sizeof(double))) return rc;
+            c->part_cap = need;
+        }
+        if (c->prec == RBMGPU_FP32) {
+            MomArgs<float> a{c->S, chunk, (int)c->P, c->n_groups, (const cplx<float>*)c->d_O, c->d_obs, c->d_part};
+            HIPCK(launch_o_partial<float>(a, (int)blocks, c->stream));
+        } else {
+            MomArgs<double> a{c->S, chunk, (int)c->P, c->n_groups, (const cplx<double>*)c->d_O, c->d_obs,
+                              c->d_part};
+            HIPCK(launch_o_partial<double>(a, (int)blocks, c->stream));
+        }
+        HIPCK(hipMemsetAsync(c->d_maxvar, 0, sizeof(double), c->stream));
+        hipLaunchKernelGGL(k_o_final, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->d_part,
+                           blocks, c->S, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
+                           (unsigned long long*)c->d_maxvar);
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipEventRecord(c->ev[5], c->stream));
+    c->moments_valid = true;
+    return 0;
+}
+
+int ensure_eval(rbmgpu_ctx* c, int64_t n) {
+    rbmgpu_ctx* ctx = c;
+    if (n <= c->ev_cap) return 0;
+    if (int rc = dalloc(ctx, &c->d_evst, n * c->nw * sizeof(uint64_t))) return rc;
+    if (int rc = dalloc(ctx, &c->d_evout, n * 2 * sizeof(double))) return rc;
+    c->ev_cap = n;
+    return 0;
+}
+
+template <typename R>
+int run_eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi, double* obs, void* Tout,
+             double* ratio, const int32_t* flips, int nf) {
+    rbmgpu_ctx* ctx = c;
+    EvalArgs<R> a{};
+    a.d = dims(c);
+    a.t = tabs<R>(c);
+    a.T = terms(c);
+    a.n = n;
+    a.states = d_states;
+    a.logpsi = logpsi;
+    a.obs = obs;
+    a.Tout = (cplx<R>*)Tout;
+    a.ratio = ratio;
+    a.n_flips = nf;
+    for (int i = 0; i < nf; i++) a.flips[i] = flips[i];
+    Shape s = c->shape.L ? c->shape : Shape{0, 0};
+    if (!s.L) {
+        for (int i = 0; i < kNumShapes; i++)
+            if (kShapes[i].L == 64 && shape_fits(c, kShapes[i]) && (!s.L || kShapes[i].K < s.K)) s = kShapes[i];
+        if (!s.L)
+            for (int i = 0; i < kNumShapes; i++)
+                if (shape_fits(c, kShapes[i]) && (!s.L || kShapes[i].L * kShapes[i].K < s.L * s.K)) s = kShapes[i];
+    }
+    if (!s.L) return fail(c, "no launch shape for this number of hidden units");
+    int block = pick_block(c, s.L, n);
+    HIPCK(launch_eval<R>(a, s, block, c->stream));
+    return 0;
+}
+
+int eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi, double* obs, void* Tout,
+         double* ratio, const int32_t* flips, int nf) {
+    return c->prec == RBMGPU_FP32 ? run_eval<float>(c, d_states, n, logpsi, obs, Tout, ratio, flips, nf)
+                                  : run_eval<double>(c, d_states, n, logpsi, obs, Tout, ratio, flips, nf);
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+const char* rbmgpu_last_error(const rbmgpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    rbmgpu_ctx* ctx = new rbmgpu_ctx();
+    rbmgpu_ctx* c = ctx;
+    auto bad = [&](const std::string& m) {
+        *out = c;  // keep the context so the caller can read the message
+        return fail(c, m);
+    };
+    c->N = cfg->n_spins;
+    c->nvb = cfg->n_vb;
+    c->na = cfg->n_alpha;
+    c->nsymm = cfg->n_symm;
+    if (c->N < 1 || c->N > 128) return bad("n_spins must be in 1..128 (MAX_SPIN_SITES)");
+    if (c->nvb < 1 || c->na < 1 || c->nsymm < 1) return bad("n_vb, n_alpha and n_symm must be positive");
+    if (c->nsymm > 1 && c->N > 255) return bad("permuted sites must fit a byte");
+    c->H = c->na * c->nsymm;
+    c->nw = (c->N + 63) / 64;
+    c->P_rbm = (int64_t)c->nvb + c->na + (int64_t)c->na * c->N;
+    c->pf_rows = cfg->pf_rows;
+    if (c->pf_rows != 0) return bad("Pfaffian factor: not supported by this build of the library");
+    c->P = c->P_rbm;
+    if (!cfg->perm || !cfg->basis) return bad("perm and basis tables are required");
+    c->perm.assign(cfg->perm, cfg->perm + (int64_t)c->nsymm * c->N);
+    c->basis.assign(cfg->basis, cfg->basis + c->nvb);
+    for (int s = 0; s < c->nsymm; s++) {
+        std::vector<int> seen(c->N, 0);
+        for (int f = 0; f < c->N; f++) {
+            int v = c->perm[s * c->N + f];
+            if (v < 0 || v >= c->N || seen[v]++) return bad("perm row is not a permutation");
+        }
+    }
+    for (int b : c->basis)
+        if (b < 0 || b >= c->N) return bad("basis site out of range");
+    // ---- sampler settings
+    c->n_chains = cfg->n_chains;
+    c->n_samples = cfg->n_samples;
+    c->warmup = cfg->n_warmup;
+    c->step = cfg->step_size;
+    c->bond_flips = cfg->bond_flips;
+    c->seed = cfg->seed;
+    c->prec = cfg->precision;
+    c->lanes_req = cfg->lanes_per_chain;
+    c->keep_o = cfg->keep_o;
+    if (c->prec != RBMGPU_FP32 && c->prec != RBMGPU_FP64) return bad("precision must be 32 or 64");
+    if (c->n_chains < 1 || c->n_samples < 1 || c->step < 1 || c->warmup < 0)
+        return bad("invalid sampler settings");
+    if (cfg->n_prop_bonds > 0) {
+        if (!cfg->prop_bonds) return bad("prop_bonds missing");
+        c->pb.assign(cfg->prop_bonds, cfg->prop_bonds + 2 * cfg->n_prop_bonds);
+        for (int v : c->pb)
+            if (v < 0 || v >= c->N) return bad("proposal bond site out of range");
+    }
+    // ---- observables as sparse rows
+    c->n_terms = cfg->n_terms;
+    c->n_groups = std::max(1, cfg->n_groups);
+    std::vector<int32_t> tk, tgroup, rowbase, rowptr;
+    std::vector<uint8_t> tsites, mask;
+    std::vector<double> val;
+    int64_t so = 0, mo = 0;
+    for (int t = 0; t < c->n_terms; t++) {
+        int k = cfg->term_k[t];
+        if (k < 1 || k > 8) return bad("term with more than 8 sites");
+        int g = cfg->term_group ? cfg->term_group[t] : 0;
+        if (g < 0 || g >= c->n_groups || (t > 0 && g < tgroup.back())) return bad("term groups must be sorted");
+        tk.push_back(k);
+        tgroup.push_back(g);
+        for (int i = 0; i < 8; i++) {
+            int s = i < k ? cfg->term_sites[so + i] : 0;
+            if (s < 0 || s >= c->N) return bad("term site out of range");
+            tsites.push_back((uint8_t)s);
+        }
+        so += k;
+        int d = 1 << k;
+        rowbase.push_back((int32_t)rowptr.size());
+        for (int loc = 0; loc < d; loc++) {
+            rowptr.push_back((int32_t)mask.size());
+            for (int j = 0; j < d; j++) {
+                double re = cfg->term_mat[2 * (mo + (int64_t)loc * d + j)];
+                double im = cfg->term_mat[2 * (mo + (int64_t)loc * d + j) + 1];
+                if (re == 0 && im == 0) continue;
+                mask.push_back((uint8_t)(loc ^ j));
+                val.push_back(re);
+                val.push_back(im);
+            }
+        }
+        rowptr.push_back((int32_t)mask.size());
+        mo += (int64_t)d * d;
+    }
+    // ---- device
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    HIPCK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    int lds = 0;
+    HIPCK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    c->lds_limit = (size_t)lds;
+    HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIPCK(hipEventCreate(&e));
+    std::vector<uint8_t> perm8(c->perm.begin(), c->perm.end());
+    if (upload(ctx, &c->d_perm, perm8.data(), perm8.size())) return bad(c->err);
+    std::vector<int32_t> inv((size_t)c->nsymm * c->N), vbs((size_t)c->nsymm * c->nvb);
+    for (int s = 0; s < c->nsymm; s++) {
+        for (int f = 0; f < c->N; f++) inv[s * c->N + c->perm[s * c->N + f]] = f;
+        for (int i = 0; i < c->nvb; i++) vbs[s * c->nvb + i] = c->perm[s * c->N + c->basis[i]];
+    }
+    if (upload(ctx, &c->d_invperm, inv.data(), inv.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_vbsite, vbs.data(), vbs.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_pb, c->pb.data(), c->pb.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_tk, tk.data(), tk.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_tgroup, tgroup.data(), tgroup.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_rowbase, rowbase.data(), rowbase.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_rowptr, rowptr.data(), rowptr.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_tsites, tsites.data(), tsites.size())) return bad(c->err);
+    if (upload(ctx, &c->d_mask, mask.data(), mask.size())) return bad(c->err);
+    if (upload(ctx, &c->d_val, val.data(), val.size() * 8)) return bad(c->err);
+    if (dalloc(ctx, &c->d_acc, c->n_chains * 4) || dalloc(ctx, &c->d_prop, c->n_chains * 4)) return bad(c->err);
+    c->params.assign(c->P, 0.);
+    if (build_weight_tables(c)) return bad(c->err);
+    *out = c;
+    return 0;
+}
+
+void rbmgpu_destroy(rbmgpu_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->d_W,      c->d_C,      c->d_S,      c->d_hb,    c->d_rb,     c->d_lb,     c->d_perm,
+                    c->d_invperm, c->d_vbsite, c->d_pb,     c->d_tk,    c->d_tgroup, c->d_rowbase, c->d_rowptr,
+                    c->d_tsites, c->d_mask,   c->d_val,    c->d_states, c->d_obs,   c->d_T,      c->d_O,
+                    c->d_acc,    c->d_prop,   c->d_part,   c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
+                    c->d_emom,   c->d_x,      c->d_b,      c->d_r,     c->d_p,      c->d_Ap,     c->d_st,
+                    c->d_sxpart, c->d_evst,   c->d_evout,  c->d_evobs};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int rbmgpu_n_params(const rbmgpu_ctx* c, int64_t* n, int64_t* nn) {
+    if (!c) return -1;
+    if (n) *n = c->P;
+    if (nn) *nn = c->P_rbm;
+    return 0;
+}
+
+int rbmgpu_launch_shape(const rbmgpu_ctx* c, int32_t* o) {
+    if (!c) return -1;
+    o[0] = c->shape.L;
+    o[1] = c->shape.K;
+    o[2] = c->block;
+    return 0;
+}
+
+int rbmgpu_set_params(rbmgpu_ctx* c, const double* p) {
+    if (!c || !p) return -1;
+    for (int64_t i = 0; i < c->P; i++) c->params[i] = zd(p[2 * i], p[2 * i + 1]);
+    return build_weight_tables(c);
+}
+
+int rbmgpu_get_params(const rbmgpu_ctx* c, double* p) {
+    if (!c || !p) return -1;
+    for (int64_t i = 0; i < c->P; i++) {
+        p[2 * i] = c->params[i].real();
+        p[2 * i + 1] = c->params[i].imag();
+    }
+    return 0;
+}
+
+int rbmgpu_update_weights(rbmgpu_ctx* c, const double* dw) {
+    if (!c || !dw) return -1;
+    for (int64_t i = 0; i < c->P; i++) c->params[i] -= zd(dw[2 * i], dw[2 * i + 1]);
+    return build_weight_tables(c);
+}
+
+int rbmgpu_sample_epoch(rbmgpu_ctx* c, uint32_t epoch) {
+    if (!c) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (int rc = autotune(c)) return rc;
+    if (int rc = ensure_samples(c, c->n_samples)) return rc;
+    HIPCK(hipMemsetAsync(c->d_obs, 0, c->n_samples * 2 * c->n_groups * sizeof(double), c->stream));
+    HIPCK(hipEventRecord(c->ev[0], c->stream));
+    int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, epoch, c->shape, c->block, c->n_samples, c->warmup)
+                                    : run_sample<double>(c, epoch, c->shape, c->block, c->n_samples, c->warmup);
+    if (rc) return rc;
+    HIPCK(hipEventRecord(c->ev[1], c->stream));
+    c->S = c->n_samples;
+    c->have_chains = true;
+    rc = c->prec == RBMGPU_FP32 ? run_build_O<float>(c) : run_build_O<double>(c);
+    if (rc) return rc;
+    HIPCK(hipEventRecord(c->ev[2], c->stream));
+    c->moments_valid = false;
+    return 0;
+}
+
+int rbmgpu_load_samples(rbmgpu_ctx* c, const uint64_t* states, int64_t n) {
+    if (!c || !states || n < 1) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (int rc = ensure_samples(c, n)) return rc;
+    HIPCK(hipMemcpyAsync(c->d_states, states, n * c->nw * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemsetAsync(c->d_obs, 0, n * 2 * c->n_groups * sizeof(double), c->stream));
+    HIPCK(hipEventRecord(c->ev[0], c->stream));
+    if (int rc = eval(c, c->d_states, n, nullptr, c->d_obs, c->d_T, nullptr, nullptr, 0)) return rc;
+    HIPCK(hipEventRecord(c->ev[1], c->stream));
+    c->S = n;
+    c->have_chains = false;
+    int rc = c->prec == RBMGPU_FP32 ? run_build_O<float>(c) : run_build_O<double>(c);
+    if (rc) return rc;
+    HIPCK(hipEventRecord(c->ev[2], c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->moments_valid = false;
+    return 0;
+}
+
+int rbmgpu_reduce(rbmgpu_ctx* c, rbmgpu_moments* m, double* o_mean, double* o_var, double* eo,
+                  double* gmeans) {
+    if (!c) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (int rc = compute_moments(c)) return rc;
+    std::vector<double> e(16 + 2 * c->n_groups);
+    HIPCK(hipMemcpyAsync(e.data(), c->d_emom, e.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> acc, prop;
+    if (c->have_chains) {
+        acc.resize(c->n_chains);
+        prop.resize(c->n_chains);
+        HIPCK(hipMemcpyAsync(acc.data(), c->d_acc, c->n_chains * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipMemcpyAsync(prop.data(), c->d_prop, c->n_chains * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (c->keep_o) {
+        if (o_mean) HIPCK(hipMemcpyAsync(o_mean, c->d_omean, c->P * 16, hipMemcpyDeviceToHost, c->stream));
+        if (o_var) HIPCK(hipMemcpyAsync(o_var, c->d_ovar, c->P * 8, hipMemcpyDeviceToHost, c->stream));
+        if (eo) HIPCK(hipMemcpyAsync(eo, c->d_eo, c->P * 16, hipMemcpyDeviceToHost, c->stream));
+    } else if (o_mean || o_var || eo) {
+        return fail(c, "O statistics need keep_o");
+    }
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (m) {
+        m->e_mean[0] = e[0];
+        m->e_mean[1] = e[1];
+        m->e_var = e[2];
+        m->e_var_binned = e[3];
+        m->e_tau = e[4];
+        m->e_std = e[5];
+        int64_t nb = 50;
+        while (c->S % nb != 0) nb--;
+        m->n_bins = nb;
+        double a = 0;
+        for (int i = 0; i < (int)acc.size(); i++) a += prop[i] ? (double)acc[i] / prop[i] : 0.0;
+        m->acceptance = acc.empty() ? 0.0 : a / c->n_chains;
+    }
+    if (gmeans) std::memcpy(gmeans, e.data() + 16, 2 * c->n_groups * sizeof(double));
+    c->times[0] = elapsed(c, 0, 1);
+    c->times[1] = elapsed(c, 1, 2);
+    c->times[2] = elapsed(c, 4, 5);
+    return 0;
+}
+
+int rbmgpu_apply_S(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd) {
+    if (!c || !x || !y) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->keep_o) return fail(c, "S.x needs keep_o");
+    if (!c->moments_valid)
+        if (int rc = compute_moments(c)) return rc;
+    if (int rc = ensure_cg(c)) return rc;
+    if (int rc = ensure_sx(c)) return rc;
+    HIPCK(hipMemcpyAsync(c->d_p, x, c->P * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipEventRecord(c->ev[6], c->stream));
+    if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, nullptr)) return rc;
+    HIPCK(hipEventRecord(c->ev[7], c->stream));
+    HIPCK(hipMemcpyAsync(y, c->d_Ap, c->P * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->times[3] = elapsed(c, 6, 7);
+    return 0;
+}
+
+int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double r2, double rd,
+                    int64_t max_iter, double rtol, int64_t* itn, double* rs) {
+    if (!c || !b || !x) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->keep_o) return fail(c, "S.x needs keep_o");
+    if (!c->moments_valid)
+        if (int rc = compute_moments(c)) return rc;
+    if (int rc = ensure_cg(c)) return rc;
+    if (int rc = ensure_sx(c)) return rc;
+    // cg_solver.cpp:48-60: max_iter = min(2P, max_iter), 0 -> 2P; rtol 0 -> 1e-16
+    int64_t mi = std::min<int64_t>(2 * c->P, max_iter);
+    if (mi == 0) mi = 2 * c->P;
+    if (rtol == 0.) rtol = 1e-16;
+    HIPCK(hipMemcpyAsync(c->d_b, b, c->P * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->d_x, x, c->P * 16, hipMemcpyHostToDevice, c->stream));
+    if (int rc = sx(c, c->d_x, c->d_Ap, r1, r2, rd, nullptr)) return rc;
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1024), 0, c->stream, c->d_b, c->d_Ap, c->d_r, c->d_p, (int)c->P,
+                       c->d_st, mi, rtol);
+    HIPCK(hipGetLastError());
+    CgStatus st{};
+    int64_t launched = 0;
+    double sx_ms = 0;
+    int64_t sx_n = 0;
+    const int batch = 16;
+    while (true) {
+        int nb = (int)std::min<int64_t>(batch, mi - launched);
+        if (nb <= 0) break;
+        HIPCK(hipEventRecord(c->ev[6], c->stream));
+        for (int i = 0; i < nb; i++) {
+            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st)) return rc;
+            hipLaunchKernelGGL(k_cg_step, dim3(1), dim3(1024), 0, c->stream, c->d_x, c->d_r, c->d_p, c->d_Ap,
+                               (int)c->P, c->d_st);
+            HIPCK(hipGetLastError());
+        }
+        HIPCK(hipEventRecord(c->ev[7], c->stream));
+        HIPCK(hipMemcpyAsync(&st, c->d_st, sizeof(CgStatus), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        launched += nb;
+        int64_t done_in_batch = st.done ? std::min<int64_t>(nb, st.itn + 1 - (launched - nb)) : nb;
+        if (done_in_batch == nb) {  // whole batch did work: use it for the per-matvec time
+            sx_ms += elapsed(c, 6, 7);
+            sx_n += nb;
+        }
+        if (st.done) break;
+    }
+    HIPCK(hipMemcpyAsync(x, c->d_x, c->P * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (itn) *itn = st.itn;
+    if (rs) *rs = st.rs;
+    if (sx_n) c->times[3] = sx_ms / sx_n;
+    return 0;
+}
+
+int rbmgpu_eval_logpsi(rbmgpu_ctx* c, const uint64_t* states, int64_t n, double* out) {
+    if (!c || !states || !out || n < 1) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (int rc = ensure_eval(c, n)) return rc;
+    HIPCK(hipMemcpyAsync(c->d_evst, states, n * c->nw * 8, hipMemcpyHostToDevice, c->stream));
+    if (int rc = eval(c, c->d_evst, n, c->d_evout, nullptr, nullptr, nullptr, nullptr, 0)) return rc;
+    HIPCK(hipMemcpyAsync(out, c->d_evout, n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rbmgpu_psi_ratio(rbmgpu_ctx* c, const uint64_t* state, const int32_t* flips, int32_t nf, double* ratio) {
+    if (!c || !state || !ratio || nf < 0 || nf > 8) return -1;
+    rbmgpu_ctx* ctx = c;
+    for (int i = 0; i < nf; i++)
+        if (flips[i] < 0 || flips[i] >= c->N) return fail(c, "flip site out of range");
+    if (nf == 0) {
+        ratio[0] = 1;
+        ratio[1] = 0;
+        return 0;
+    }
+    if (int rc = ensure_eval(c, 1)) return rc;
+    HIPCK(hipMemcpyAsync(c->d_evst, state, c->nw * 8, hipMemcpyHostToDevice, c->stream));
+    if (int rc = eval(c, c->d_evst, 1, nullptr, nullptr, nullptr, c->d_evout, flips, nf)) return rc;
+    HIPCK(hipMemcpyAsync(ratio, c->d_evout, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rbmgpu_get_samples(const rbmgpu_ctx* cc, uint64_t* states, double* obs, double* O, double* chain_acc) {
+    rbmgpu_ctx* c = const_cast<rbmgpu_ctx*>(cc);
+    if (!c) return -1;
+    rbmgpu_ctx* ctx = c;
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (states) HIPCK(hipMemcpy(states, c->d_states, c->S * c->nw * 8, hipMemcpyDeviceToHost));
+    if (obs) HIPCK(hipMemcpy(obs, c->d_obs, c->S * 2 * c->n_groups * 8, hipMemcpyDeviceToHost));
+    if (O) {
+        if (!c->keep_o) return fail(c, "O not kept");
+        if (c->prec == RBMGPU_FP64) {
+            HIPCK(hipMemcpy(O, c->d_O, c->S * c->P * 16, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(c->S * c->P * 2);
+            HIPCK(hipMemcpy(f.data(), c->d_O, f.size() * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < f.size(); i++) O[i] = f[i];
+        }
+    }
+    if (chain_acc) {
+        if (!c->have_chains) return fail(c, "no sampled chains");
+        std::vector<uint32_t> a(c->n_chains), p(c->n_chains);
+        HIPCK(hipMemcpy(a.data(), c->d_acc, c->n_chains * 4, hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(p.data(), c->d_prop, c->n_chains * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->n_chains; i++) chain_acc[i] = p[i] ? (double)a[i] / p[i] : 0.0;
+    }
+    return 0;
+}
+
+int rbmgpu_n_samples(const rbmgpu_ctx* c, int64_t* n) {
+    if (!c || !n) return -1;
+    *n = c->S;
+    return 0;
+}
+
+int rbmgpu_kernel_times(const rbmgpu_ctx* c, double* ms5) {
+    if (!c || !ms5) return -1;
+    for (int i = 0; i < 5; i++) ms5[i] = c->times[i];
+    return 0;
+}
+
+int rbmgpu_synchronize(rbmgpu_ctx* c) {
+    if (!c) return -1;
+    rbmgpu_ctx* ctx = c;
+    HIPCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

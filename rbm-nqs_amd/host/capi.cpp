@@ -1,0 +1,122 @@
+// capi.cpp -- implementation of include/rbmhost.h.
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/rbmhost.h"
+#include "setup.hpp"
+
+struct rbmhost_model {
+    std::unique_ptr<rbmhost::setup> s;
+};
+
+extern "C" {
+
+rbmhost_model* rbmhost_create(const char* ini_text, char* err, int32_t err_len) {
+    try {
+        ini::config cfg;
+        ini::parse_ini_text(ini_text ? ini_text : "", cfg);
+        ini::finalize(cfg, !cfg.name.empty());
+        auto* m = new rbmhost_model();
+        m->s.reset(new rbmhost::setup(cfg));
+        return m;
+    } catch (const std::exception& e) {
+        if (err && err_len > 0) {
+            std::strncpy(err, e.what(), err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return nullptr;
+    }
+}
+
+void rbmhost_destroy(rbmhost_model* m) { delete m; }
+
+int rbmhost_info(const rbmhost_model* m, int64_t* o) {
+    if (!m || !o) return -1;
+    const auto& s = *m->s;
+    const auto& L = s.layout();
+    const auto& cfg = s.cfg();
+    o[0] = L.n_spins;
+    o[1] = L.n_params();
+    o[2] = L.n_neural();
+    o[3] = L.n_vb;
+    o[4] = L.n_alpha;
+    o[5] = L.n_symm;
+    o[6] = L.pf_rows;
+    o[7] = (int64_t)s.terms().k.size();
+    o[8] = s.terms().n_groups;
+    o[9] = (int64_t)m->s->model().get_lattice().get_bonds().size();
+    o[10] = (int64_t)cfg.sa_metropolis_n_chains;
+    o[11] = (int64_t)s.n_samples();
+    o[12] = (int64_t)s.warmup();
+    o[13] = (int64_t)cfg.sa_metropolis_n_steps_per_sample;
+    o[14] = (int64_t)s.prop_bonds().size() / 2;
+    return 0;
+}
+
+int rbmhost_bonds(const rbmhost_model* m, int64_t* out) {
+    if (!m || !out) return -1;
+    size_t i = 0;
+    for (const auto& b : m->s->model().get_lattice().get_bonds()) {
+        out[i++] = (int64_t)b.a;
+        out[i++] = (int64_t)b.b;
+        out[i++] = (int64_t)b.type;
+    }
+    return 0;
+}
+
+int rbmhost_perms(const rbmhost_model* m, int64_t* out) {
+    if (!m || !out) return -1;
+    const auto& p = m->s->layout().perm;
+    for (size_t i = 0; i < p.size(); i++) out[i] = p[i];
+    return 0;
+}
+
+int rbmhost_terms(const rbmhost_model* m, int64_t* k, int64_t* sites, double* mats, int64_t* groups) {
+    if (!m) return -1;
+    const auto& t = m->s->terms();
+    for (size_t i = 0; i < t.k.size(); i++) {
+        if (k) k[i] = t.k[i];
+        if (groups) groups[i] = t.group[i];
+    }
+    if (sites)
+        for (size_t i = 0; i < t.sites.size(); i++) sites[i] = t.sites[i];
+    if (mats) std::memcpy(mats, t.mat.data(), t.mat.size() * sizeof(double));
+    return 0;
+}
+
+double rbmhost_exact_energy(const rbmhost_model* m) {
+    if (!m) return 0;
+    if (m->s->cfg().model == "kitaev") return static_cast<model::kitaev&>(m->s->model()).exact_energy();
+    return m->s->cfg().J.at(0);
+}
+
+int rbmhost_use_evaluation(rbmhost_model* m, int32_t hexagons) {
+    if (!m) return -1;
+    m->s->use_evaluation_observables(hexagons != 0);
+    return 0;
+}
+
+int rbmhost_set_n_samples(rbmhost_model* m, int64_t n) {
+    if (!m || n < 1) return -1;
+    m->s->set_n_samples((size_t)n);
+    return 0;
+}
+
+int rbmhost_initial_weights(const rbmhost_model* m, uint64_t seed, double* params) {
+    if (!m || !params) return -1;
+    std::mt19937 rng((std::mt19937::result_type)seed);
+    auto p = m->s->initial_weights(rng);
+    std::memcpy(params, p.data(), p.size() * sizeof(std::complex<double>));
+    return 0;
+}
+
+int rbmhost_gpu_create(const rbmhost_model* m, int32_t precision, int32_t lanes, rbmgpu_ctx** out) {
+    if (!m || !out) return -1;
+    rbmgpu_config c = m->s->gpu_config();
+    if (precision) c.precision = precision;
+    if (lanes) c.lanes_per_chain = lanes;
+    return rbmgpu_create(&c, out);
+}
+
+}  // extern "C"

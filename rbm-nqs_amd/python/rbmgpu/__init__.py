@@ -1,0 +1,279 @@
+"""ctypes binding of the MI355X VMC library (include/rbmgpu.h, include/rbmhost.h).
+
+`Model` builds lattice, Hamiltonian and machine tables from the reference's
+.ini text; `Gpu` owns one device context.  The native libraries are loaded
+from rbm-nqs_amd/lib; there is no fallback: a missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(os.path.dirname(os.path.dirname(_PKG)), "lib")
+_gpu = None
+_host = None
+
+DP = C.POINTER(C.c_double)
+I64P = C.POINTER(C.c_int64)
+I32P = C.POINTER(C.c_int32)
+U64P = C.POINTER(C.c_uint64)
+
+FP32, FP64 = 32, 64
+
+
+class Moments(C.Structure):
+    _fields_ = [("e_mean", C.c_double * 2), ("e_var", C.c_double), ("e_var_binned", C.c_double),
+                ("e_tau", C.c_double), ("e_std", C.c_double), ("n_bins", C.c_int64),
+                ("acceptance", C.c_double)]
+
+
+def libs():
+    """Load librbmgpu.so and librbmhost.so (raises if they are not built)."""
+    global _gpu, _host
+    if _gpu is None:
+        g = os.path.join(LIBDIR, "librbmgpu.so")
+        h = os.path.join(LIBDIR, "librbmhost.so")
+        if not (os.path.exists(g) and os.path.exists(h)):
+            raise RuntimeError(f"native libraries missing in {LIBDIR}: run rbm-nqs_amd/build.py")
+        _gpu = C.CDLL(g, mode=C.RTLD_GLOBAL)
+        _host = C.CDLL(h)
+        _gpu.rbmgpu_last_error.restype = C.c_char_p
+        _gpu.rbmgpu_last_error.argtypes = [C.c_void_p]
+        _gpu.rbmgpu_destroy.argtypes = [C.c_void_p]
+        _gpu.rbmgpu_destroy.restype = None
+        _host.rbmhost_create.restype = C.c_void_p
+        _host.rbmhost_create.argtypes = [C.c_char_p, C.c_char_p, C.c_int32]
+        _host.rbmhost_destroy.argtypes = [C.c_void_p]
+        _host.rbmhost_destroy.restype = None
+        _host.rbmhost_exact_energy.restype = C.c_double
+        _host.rbmhost_exact_energy.argtypes = [C.c_void_p]
+    return _gpu, _host
+
+
+def exported_symbols():
+    """Names the two libraries must export (checked by the CPU tests)."""
+    return {
+        "gpu": ["rbmgpu_create", "rbmgpu_destroy", "rbmgpu_last_error", "rbmgpu_n_params",
+                "rbmgpu_launch_shape", "rbmgpu_set_params", "rbmgpu_get_params",
+                "rbmgpu_update_weights", "rbmgpu_sample_epoch", "rbmgpu_load_samples",
+                "rbmgpu_reduce", "rbmgpu_sr_solve", "rbmgpu_apply_S", "rbmgpu_eval_logpsi",
+                "rbmgpu_psi_ratio", "rbmgpu_get_samples", "rbmgpu_n_samples",
+                "rbmgpu_kernel_times", "rbmgpu_synchronize"],
+        "host": ["rbmhost_create", "rbmhost_destroy", "rbmhost_info", "rbmhost_bonds",
+                 "rbmhost_perms", "rbmhost_terms", "rbmhost_exact_energy", "rbmhost_use_evaluation",
+                 "rbmhost_set_n_samples", "rbmhost_initial_weights", "rbmhost_gpu_create"],
+    }
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Model:
+    """Host tables of one .ini configuration (rbmhost_*)."""
+
+    def __init__(self, ini_text):
+        _, h = libs()
+        err = C.create_string_buffer(512)
+        self._h = h.rbmhost_create(ini_text.encode(), err, 512)
+        if not self._h:
+            raise RuntimeError("rbmhost: " + err.value.decode())
+        self._refresh()
+
+    def _refresh(self):
+        _, h = libs()
+        o = np.zeros(15, dtype=np.int64)
+        h.rbmhost_info(C.c_void_p(self._h), _p(o, I64P))
+        (self.N, self.P, self.n_neural, self.n_vb, self.n_alpha, self.n_symm, self.pf_rows,
+         self.n_terms, self.n_groups, self.n_bonds, self.n_chains, self.n_samples, self.n_warmup,
+         self.step_size, self.n_prop_bonds) = [int(x) for x in o]
+        self.n_words = (self.N + 63) // 64
+        self.H = self.n_alpha * self.n_symm
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            libs()[1].rbmhost_destroy(C.c_void_p(self._h))
+            self._h = None
+
+    @property
+    def h(self):
+        return C.c_void_p(self._h)
+
+    def bonds(self):
+        out = np.zeros(3 * self.n_bonds, dtype=np.int64)
+        libs()[1].rbmhost_bonds(self.h, _p(out, I64P))
+        return out.reshape(-1, 3)
+
+    def perms(self):
+        out = np.zeros(self.n_symm * self.N, dtype=np.int64)
+        libs()[1].rbmhost_perms(self.h, _p(out, I64P))
+        return out.reshape(self.n_symm, self.N)
+
+    def terms(self):
+        k = np.zeros(self.n_terms, dtype=np.int64)
+        g = np.zeros(self.n_terms, dtype=np.int64)
+        libs()[1].rbmhost_terms(self.h, _p(k, I64P), None, None, _p(g, I64P))
+        sites = np.zeros(int(k.sum()), dtype=np.int64)
+        mats = np.zeros(int(np.sum(4 ** k)) * 2)
+        libs()[1].rbmhost_terms(self.h, None, _p(sites, I64P), _p(mats, DP), None)
+        out, so, mo = [], 0, 0
+        mz = mats.view(np.complex128)
+        for t in range(self.n_terms):
+            d = 1 << int(k[t])
+            out.append((sites[so: so + k[t]], mz[mo: mo + d * d].reshape(d, d), int(g[t])))
+            so += k[t]
+            mo += d * d
+        return out
+
+    def exact_energy(self):
+        return libs()[1].rbmhost_exact_energy(self.h)
+
+    def use_evaluation(self, hexagons=True):
+        libs()[1].rbmhost_use_evaluation(self.h, C.c_int32(int(hexagons)))
+        self._refresh()
+
+    def set_n_samples(self, n):
+        libs()[1].rbmhost_set_n_samples(self.h, C.c_int64(n))
+        self._refresh()
+
+    def initial_weights(self, seed):
+        p = np.zeros(self.P, dtype=np.complex128)
+        libs()[1].rbmhost_initial_weights(self.h, C.c_uint64(seed), _p(p, DP))
+        return p
+
+    def gpu(self, precision=0, lanes=0):
+        return Gpu(self, precision, lanes)
+
+
+class Gpu:
+    """One device context (rbmgpu_*)."""
+
+    def __init__(self, model, precision=0, lanes=0):
+        g, h = libs()
+        ctx = C.c_void_p()
+        rc = h.rbmhost_gpu_create(model.h, C.c_int32(precision), C.c_int32(lanes), C.byref(ctx))
+        self._c = ctx.value
+        if rc != 0:
+            msg = g.rbmgpu_last_error(C.c_void_p(self._c)).decode() if self._c else "rbmgpu_create failed"
+            if self._c:
+                g.rbmgpu_destroy(C.c_void_p(self._c))
+            self._c = None
+            raise RuntimeError("rbmgpu: " + msg)
+        self.model = model
+        n = np.zeros(2, dtype=np.int64)
+        g.rbmgpu_n_params(self.c, _p(n[:1], I64P), _p(n[1:], I64P))
+        self.P, self.n_neural = int(n[0]), int(n[1])
+        self.N = model.N
+        self.n_words = model.n_words
+        self.n_groups = model.n_groups
+        self.n_chains = model.n_chains
+
+    def __del__(self):
+        if getattr(self, "_c", None):
+            libs()[0].rbmgpu_destroy(C.c_void_p(self._c))
+            self._c = None
+
+    @property
+    def c(self):
+        return C.c_void_p(self._c)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError("rbmgpu: " + libs()[0].rbmgpu_last_error(self.c).decode())
+
+    def launch_shape(self):
+        o = np.zeros(3, dtype=np.int32)
+        self._chk(libs()[0].rbmgpu_launch_shape(self.c, _p(o, I32P)))
+        return tuple(int(x) for x in o)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.complex128)
+        assert p.size == self.P
+        self._chk(libs()[0].rbmgpu_set_params(self.c, _p(p, DP)))
+
+    def get_params(self):
+        p = np.zeros(self.P, dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_get_params(self.c, _p(p, DP)))
+        return p
+
+    def update_weights(self, dw):
+        dw = np.ascontiguousarray(dw, dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_update_weights(self.c, _p(dw, DP)))
+
+    def sample_epoch(self, epoch):
+        self._chk(libs()[0].rbmgpu_sample_epoch(self.c, C.c_uint32(epoch)))
+
+    def load_samples(self, states):
+        s = np.ascontiguousarray(states, dtype=np.uint64).reshape(-1, self.n_words)
+        self._chk(libs()[0].rbmgpu_load_samples(self.c, _p(s, U64P), C.c_int64(s.shape[0])))
+
+    def n_samples(self):
+        n = C.c_int64()
+        self._chk(libs()[0].rbmgpu_n_samples(self.c, C.byref(n)))
+        return n.value
+
+    def reduce(self, want_o=True):
+        m = Moments()
+        P = self.P
+        om = np.zeros(P, dtype=np.complex128) if want_o else None
+        ov = np.zeros(P) if want_o else None
+        eo = np.zeros(P, dtype=np.complex128) if want_o else None
+        gm = np.zeros(self.n_groups, dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_reduce(self.c, C.byref(m), _p(om, DP), _p(ov, DP), _p(eo, DP),
+                                          _p(gm, DP)))
+        res = dict(mean=complex(m.e_mean[0], m.e_mean[1]), var=m.e_var, var_binned=m.e_var_binned,
+                   tau=m.e_tau, std=m.e_std, n_bins=m.n_bins, acc=m.acceptance, group_means=gm)
+        if want_o:
+            res.update(o_mean=om, o_var=ov, eo=eo)
+        return res
+
+    def apply_S(self, x, r1, r2, rd):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.P, dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_apply_S(self.c, _p(x, DP), _p(y, DP), C.c_double(r1), C.c_double(r2),
+                                           C.c_double(rd)))
+        return y
+
+    def sr_solve(self, b, x0, r1, r2, rd, max_iter=0, rtol=0.0):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.array(x0, dtype=np.complex128)
+        itn = C.c_int64()
+        rs = C.c_double()
+        self._chk(libs()[0].rbmgpu_sr_solve(self.c, _p(b, DP), _p(x, DP), C.c_double(r1), C.c_double(r2),
+                                            C.c_double(rd), C.c_int64(max_iter), C.c_double(rtol),
+                                            C.byref(itn), C.byref(rs)))
+        return x, itn.value, rs.value
+
+    def eval_logpsi(self, states):
+        s = np.ascontiguousarray(states, dtype=np.uint64).reshape(-1, self.n_words)
+        out = np.zeros(s.shape[0], dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_eval_logpsi(self.c, _p(s, U64P), C.c_int64(s.shape[0]), _p(out, DP)))
+        return out
+
+    def psi_ratio(self, state, flips):
+        s = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
+        f = np.ascontiguousarray(flips, dtype=np.int32)
+        out = np.zeros(1, dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_psi_ratio(self.c, _p(s, U64P), _p(f, I32P), C.c_int32(f.size),
+                                             _p(out, DP)))
+        return out[0]
+
+    def get_samples(self, states=True, obs=True, O=False, chain_acc=False):
+        S = self.n_samples()
+        st = np.zeros((S, self.n_words), dtype=np.uint64) if states else None
+        ob = np.zeros((S, self.n_groups), dtype=np.complex128) if obs else None
+        Om = np.zeros(S * self.P, dtype=np.complex128) if O else None
+        ca = np.zeros(self.n_chains) if chain_acc else None
+        self._chk(libs()[0].rbmgpu_get_samples(self.c, _p(st, U64P), _p(ob, DP), _p(Om, DP), _p(ca, DP)))
+        if Om is not None:
+            Om = Om.reshape(S, self.P).T
+        return st, ob, Om, ca
+
+    def kernel_times(self):
+        t = np.zeros(5)
+        self._chk(libs()[0].rbmgpu_kernel_times(self.c, _p(t, DP)))
+        return dict(zip(["sample", "assemble", "reduce", "sx", "cg"], t.tolist()))
+
+    def synchronize(self):
+        self._chk(libs()[0].rbmgpu_synchronize(self.c))

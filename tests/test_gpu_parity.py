@@ -1,0 +1,177 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the FP64 CPU
+oracle on the same inputs.
+
+Tolerances: FP64 mode 1e-10 relative; FP32 mode as stated
+per test (FP32 arithmetic, FP64 accumulation of observables).  Imaginary
+parts of log psi are compared modulo 2 pi (sum of principal arguments).
+FP32 ratios carry their rounding in the log-ratio, so their relative
+tolerance is 2e-5 * (1 + |ln|ratio||).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import rbmgpu
+from helpers import ini, random_params
+
+pytestmark = pytest.mark.gpu
+
+C1 = ini("kitaev", 3, extra="[sampler]\nn_samples = 1000\nmetropolis.n_chains = 4\n")
+C2_SMALL = ini("toric", 4, alpha=2, extra="[sampler]\nn_samples = 128\nmetropolis.n_chains = 16\n")
+C3_SMALL = ini("kitaev", 6, rbm="symmetry", alpha=4,
+               extra="[sampler]\nn_samples = 64\nmetropolis.n_chains = 16\n")
+SYM_TORIC = ini("toric", 3, rbm="symmetry", alpha=2, extra="[sampler]\nn_samples = 64\nmetropolis.n_chains = 8\n")
+SPECS = {"C1": C1, "C2": C2_SMALL, "C3": C3_SMALL, "symtoric": SYM_TORIC}
+
+
+def _wrap(x):
+    return (x + np.pi) % (2 * np.pi) - np.pi
+
+
+def _setup(spec, prec, std=0.3, seed=1):
+    print(f"[gpu-test] spec={spec.split()[3:6]} prec={prec} std={std}", flush=True)
+    rng = np.random.default_rng(seed)
+    m = rbmgpu.Model(spec)
+    o = po.Oracle(spec)
+    p = random_params(rng, m.P, std)
+    o.set_params(p)
+    g = m.gpu(precision=prec)
+    g.set_params(p)
+    return m, o, g, rng
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("prec", [64, 32])
+def test_logpsi(name, prec):
+    m, o, g, rng = _setup(SPECS[name], prec)
+    states = po.random_states(rng, 64, m.N)
+    a = g.eval_logpsi(states)
+    b = o.logpsi(states)
+    tol = 1e-10 if prec == 64 else 2e-5
+    scale = 1 + np.abs(b.real)
+    assert np.all(np.abs(a.real - b.real) <= tol * scale), np.max(np.abs(a.real - b.real) / scale)
+    assert np.all(np.abs(_wrap(a.imag - b.imag)) <= tol * scale)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("prec", [64, 32])
+def test_psi_ratio(name, prec):
+    m, o, g, rng = _setup(SPECS[name], prec)
+    for _ in range(8):
+        s = po.random_states(rng, 1, m.N)
+        f = rng.choice(m.N, int(rng.integers(1, 7)), replace=False)
+        a = g.psi_ratio(s, f)
+        b = o.psi_ratio(s, f)
+        tol = 1e-10 if prec == 64 else 2e-5 * (1 + abs(np.log(abs(b))))
+        assert abs(a - b) <= tol * abs(b), (a, b)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("prec", [64, 32])
+def test_eloc_and_O(name, prec):
+    m, o, g, rng = _setup(SPECS[name], prec, std=0.3 if prec == 64 else 0.1)
+    states = po.random_states(rng, 96, m.N)
+    g.load_samples(states)
+    st, obs, O, _ = g.get_samples(states=True, obs=True, O=True)
+    assert np.array_equal(st, states)
+    e_ref = o.eloc(states)
+    O_ref = o.derivative(states)
+    tol = 1e-10 if prec == 64 else 1e-4
+    escale = np.abs(e_ref).max() + 1
+    assert np.max(np.abs(obs[:, 0] - e_ref)) <= tol * escale
+    oscale = np.abs(O_ref).max()
+    assert np.max(np.abs(O - O_ref)) <= tol * oscale
+
+
+@pytest.mark.parametrize("name", ["C1", "C2", "C3"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_reduce_S_and_cg(name, prec):
+    m, o, g, rng = _setup(SPECS[name], prec, std=0.2)
+    S = 200
+    states = po.random_states(rng, S, m.N)
+    g.load_samples(states)
+    e = o.eloc(states)
+    O = o.derivative(states)
+    ref = o.aggregate(e, O)
+    got = g.reduce()
+    tol = 1e-10 if prec == 64 else 1e-5
+    assert abs(got["mean"] - ref["mean"]) <= tol * (1 + abs(ref["mean"]))
+    for k in ("var", "var_binned", "tau"):
+        assert abs(got[k] - ref[k]) <= 10 * tol * (1 + abs(ref[k])), k
+    assert np.allclose(got["o_mean"], ref["o_mean"], rtol=0, atol=tol * (1 + np.abs(ref["o_mean"]).max()))
+    assert np.allclose(got["o_var"], ref["o_var"], rtol=10 * tol, atol=10 * tol)
+    assert np.allclose(got["eo"], ref["eo"], rtol=0, atol=tol * (1 + np.abs(ref["eo"]).max()))
+    # S.x on the same sample set (cg_solver.cpp:301-317)
+    x = rng.normal(size=m.P) + 1j * rng.normal(size=m.P)
+    r1, r2, rd = 0.9, 9e-4, 1e-2
+    y = g.apply_S(x, r1, r2, rd)
+    y_ref = po.apply_S(O, ref["o_mean"], ref["o_var"], r1, r2, rd, m.n_neural, x)
+    assert np.max(np.abs(y - y_ref)) <= tol * np.abs(y_ref).max()
+    # CG with a fixed iteration count and warm start (cg_solver.cpp:119-189)
+    F = ref["eo"] - ref["mean"] * ref["o_mean"].conj()
+    x0 = 0.01 * (rng.normal(size=m.P) + 1j * rng.normal(size=m.P))
+    xg, itg, rsg = g.sr_solve(F, x0, r1, r2, rd, max_iter=25, rtol=1e-300)
+    xo, ito, rso = po.cg(O, ref["o_mean"], ref["o_var"], r1, r2, rd, m.n_neural, F, x0, 25, 1e-300)
+    assert itg == ito == 25
+    ctol = 1e-8 if prec == 64 else 1e-3
+    assert np.max(np.abs(xg - xo)) <= ctol * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("name", ["C1", "C2", "symtoric", "C3"])
+def test_sampler_trajectory_matches_oracle_fp64(name):
+    """Same Philox streams and proposal rules: in FP64 the GPU chains follow
+    the oracle's chains state for state."""
+    m, o, g, rng = _setup(SPECS[name], 64, std=0.05)
+    g.sample_epoch(epoch=3)
+    st, obs, _, acc = g.get_samples(states=True, obs=True, O=False, chain_acc=True)
+    S = m.n_samples
+    st_o, e_o, _, acc_o = o.sample(seed=421390484, epoch=3, n_samples=S, n_chains=m.n_chains)
+    same = np.all(st == st_o, axis=1)
+    assert same.mean() == 1.0, f"{(~same).sum()} of {S} samples differ"
+    assert np.max(np.abs(obs[:, 0] - e_o)) <= 1e-10 * (np.abs(e_o).max() + 1)
+    assert np.allclose(acc, acc_o, rtol=0, atol=1e-15)
+
+
+def test_sampler_fp32_matches_exact_distribution():
+    spec = ini("kitaev", 2, extra="""
+[sampler]
+n_samples = 65536
+metropolis.n_chains = 256
+metropolis.n_warmup_steps = 50
+""")
+    m, o, g, rng = _setup(spec, 32, std=0.4, seed=12)
+    exact = o.exact_expectation()
+    means = []
+    for ep in range(4):
+        g.sample_epoch(epoch=ep)
+        r = g.reduce(want_o=False)
+        means.append(r["mean"].real)
+        std = r["std"]
+    est = np.mean(means)
+    assert abs(est - exact.real) < 5 * std / 2 + 1e-4, (est, exact, std)
+
+
+def test_epoch_fp64_matches_oracle_epoch():
+    """sample -> reduce -> F -> CG -> update, GPU vs oracle (SR.cpp:81-116)."""
+    spec = ini("toric", 3, alpha=1, extra="""
+[sampler]
+n_samples = 400
+metropolis.n_chains = 8
+[optimizer]
+sr.max_iterations = 50
+""")
+    m, o, g, rng = _setup(spec, 64, std=0.05)
+    p0 = o.get_params()
+    st = o.epoch(seed=421390484, epoch=0)  # oracle: lr=1e-2, reg1=0.9, reg2=9e-4 at epoch 1
+    g.sample_epoch(epoch=0)
+    r = g.reduce()
+    F = r["eo"] - r["mean"] * r["o_mean"].conj()
+    dw, itn, rs = g.sr_solve(F, np.zeros(m.P), 0.9, 9e-4, 1e-2, max_iter=50)
+    assert itn == st["itn"]
+    assert abs(r["mean"].real / m.N - st["energy"]) < 1e-12
+    g.update_weights(1e-2 * dw)
+    p_gpu = g.get_params()
+    p_or = o.get_params()
+    assert np.max(np.abs(p_gpu - p_or)) <= 1e-9 * np.abs(p_or - p0).max()
