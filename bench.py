@@ -27,7 +27,7 @@ METRIC = "samples/s and SR epoch ms at fixed energy error, plain vs learned prop
 HBM_PEAK_GBS = 8000.0
 
 WORKLOAD = dict(model="toric", n_cells=4, rbm="basic", alpha=2, chains=4096, samples_per_chain=64,
-                warmup=100, bond_flips=0.5, sr_max_iterations=100, precision=32)
+                warmup=100, bond_flips=0.5, sr_max_iterations=100, precision=32, weights_std=0.05)
 
 
 def spec(chains, samples_per_chain, warmup=100, max_iter=100):
@@ -39,6 +39,7 @@ J = -1
 [rbm]
 type = basic
 alpha = 2
+weights = 0.05
 [sampler]
 n_samples = {chains * samples_per_chain}
 metropolis.n_chains = {chains}
@@ -107,7 +108,7 @@ def cpu_baseline():
     chains, spc = 64, 32
     o = po.Oracle(spec(chains, spc))
     rng = np.random.default_rng(421390484)
-    o.set_params(1e-4 * (rng.normal(size=o.P) + 1j * rng.normal(size=o.P)))
+    o.set_params(WORKLOAD["weights_std"] * (rng.normal(size=o.P) + 1j * rng.normal(size=o.P)))
     threads = os.cpu_count() or 1
     t0 = time.perf_counter()
     st = o.epoch(seed=421390484, epoch=0, threads=threads, use_lut=True)
@@ -155,8 +156,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "toric code 4x4 (N=32), basic RBM alpha=2 (P=2144), 4096 chains x 64 samples, "
-                        "SR+CG (100 iterations, FP64 vectors); one step = one SR epoch",
+            "workload": "toric code 4x4 (N=32), basic RBM alpha=2 (P=2144), weights N(0,0.05^2) re+im, "
+                        "4096 chains x 64 samples, 100 warm-up sweeps, bond_flips 0.5, SR+CG (max 100 "
+                        "iterations, FP64 vectors); one step = one SR epoch",
             "samples_per_epoch": S,
             "parallelism": "single GPU",
         },

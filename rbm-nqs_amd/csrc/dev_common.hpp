@@ -53,6 +53,7 @@ struct M<double> {
     static __device__ double exp(double x) { return ::exp(x); }
     static __device__ double log(double x) { return ::log(x); }
     static __device__ double log1p(double x) { return ::log1p(x); }
+    static __device__ double expm1(double x) { return ::expm1(x); }
     static __device__ double atan2(double y, double x) { return ::atan2(y, x); }
     static __device__ void sincos(double x, double* s, double* c) { ::sincos(x, s, c); }
     static __device__ double fabs(double x) { return ::fabs(x); }
@@ -65,6 +66,7 @@ struct M<float> {
     static __device__ float exp(float x) { return __expf(x); }
     static __device__ float log(float x) { return __logf(x); }
     static __device__ float log1p(float x) { return ::log1pf(x); }
+    static __device__ float expm1(float x) { return ::expm1f(x); }
     static __device__ float atan2(float y, float x) { return ::atan2f(y, x); }
     static __device__ void sincos(float x, float* s, float* c) { ::sincosf(x, s, c); }
     static __device__ float fabs(float x) { return ::fabsf(x); }
@@ -73,17 +75,22 @@ struct M<float> {
     static __device__ float ldexp(float x, int e) { return ::ldexpf(x, e); }
 };
 
-// tanh(x + iy) = [sgn(x)(1 - E^2) + i 2E sin 2y] / (1 + E^2 + 2E cos 2y),
-// E = exp(-2|x|): multiply numerator and denominator of
-// (sinh 2x + i sin 2y) / (cosh 2x + cos 2y) by 2E; finite for any x.
+// tanh(x + iy) = (sinh 2x + i sin 2y) / (cosh 2x + cos 2y); dividing by
+// 2 cosh^2 x gives (th + i s c sech^2 x) / (c^2 + th^2 s^2) with th = tanh x,
+// s = sin y, c = cos y: the denominator is a sum of non-negative terms (no
+// cancellation near the poles x = 0, y = pi/2 + k pi) and nothing overflows.
+// th and sech^2 come from em = expm1(-2|x|): th = -em / (2 + em),
+// sech^2 = 4 (1 + em) / (2 + em)^2.
 template <typename R>
 __device__ inline cplx<R> ctanh(cplx<R> z) {
-    R E = M<R>::exp(-2 * M<R>::fabs(z.re));
+    R em = M<R>::expm1(-2 * M<R>::fabs(z.re));
+    R q = 1 / (2 + em);
+    R th = M<R>::copysign(-em * q, z.re);
+    R sech2 = 4 * (1 + em) * q * q;
     R s, c;
-    M<R>::sincos(2 * z.im, &s, &c);
-    R den = 1 + E * E + 2 * E * c;
-    R inv = 1 / den;
-    return {M<R>::copysign(1 - E * E, z.re) * inv, 2 * E * s * inv};
+    M<R>::sincos(z.im, &s, &c);
+    R inv = 1 / (c * c + th * th * s * s);
+    return {th * inv, s * c * sech2 * inv};
 }
 
 // ln cosh(x + iy) = ln cosh x + ln(cos y + i tanh(x) sin y), with
@@ -92,12 +99,12 @@ __device__ inline cplx<R> ctanh(cplx<R> z) {
 template <typename R>
 __device__ inline cplx<R> clncosh(cplx<R> z) {
     R a = M<R>::fabs(z.re);
-    R E = M<R>::exp(-2 * a);
-    R th = M<R>::copysign((1 - E) / (1 + E), z.re);
+    R em = M<R>::expm1(-2 * a);  // E - 1 without cancellation
+    R th = M<R>::copysign(-em / (2 + em), z.re);
     R s, c;
     M<R>::sincos(z.im, &s, &c);
     s *= th;
-    R re = a - (R)0.69314718055994530942 + M<R>::log1p(E) + (R)0.5 * M<R>::log(s * s + c * c);
+    R re = a - (R)0.69314718055994530942 + M<R>::log1p(1 + em) + (R)0.5 * M<R>::log(s * s + c * c);
     return {re, M<R>::atan2(s, c)};
 }
 

@@ -71,6 +71,8 @@ struct rbmgpu_ctx {
     // timing
     hipEvent_t ev[12] = {};
     double times[5] = {0, 0, 0, 0, 0};
+    std::vector<char> staging;
+    hipEvent_t sxev[34] = {};
     std::string err;
 };
 
@@ -132,17 +134,39 @@ int build_weight_tables(rbmgpu_ctx* c) {
     for (int f = 0; f < N; f++) rb[f] = c->params[f % nvb];
     for (int i = 0; i < nvb; i++)
         for (int s = 0; s < c->nsymm; s++) lb[c->perm[s * N + c->basis[i]]] += c->params[i];
-    auto put = [&](void** dst, const std::vector<zd>& v) -> int {
-        if (c->prec == RBMGPU_FP64) return upload(ctx, dst, v.data(), v.size() * sizeof(zd));
-        std::vector<std::complex<float>> f(v.begin(), v.end());
-        return upload(ctx, dst, f.data(), f.size() * sizeof(std::complex<float>));
+    // one staging block [W | C | S | hb | rb | lb] copied asynchronously on
+    // the context stream; the staging buffer lives in the context so the
+    // copy may complete after this call returns
+    const size_t nwt = (size_t)N * na, total = 3 * nwt + na + 2 * (size_t)N;
+    const size_t rs = rsize(c) * 2;
+    if (!c->d_W) {
+        char* base = nullptr;
+        HIPCK(hipMalloc((void**)&base, total * rs));
+        c->d_W = base;
+        c->d_C = base + nwt * rs;
+        c->d_S = base + 2 * nwt * rs;
+        c->d_hb = base + 3 * nwt * rs;
+        c->d_rb = base + (3 * nwt + na) * rs;
+        c->d_lb = base + (3 * nwt + na + N) * rs;
+    }
+    HIPCK(hipStreamSynchronize(c->stream));  // previous copy out of the staging block
+    c->staging.resize(total * rs);
+    auto pack = [&](size_t off, const std::vector<zd>& v) {
+        for (size_t i = 0; i < v.size(); i++) {
+            if (c->prec == RBMGPU_FP64) {
+                reinterpret_cast<zd*>(c->staging.data())[off + i] = v[i];
+            } else {
+                reinterpret_cast<std::complex<float>*>(c->staging.data())[off + i] = std::complex<float>(v[i]);
+            }
+        }
     };
-    if (int rc = put(&c->d_W, W)) return rc;
-    if (int rc = put(&c->d_C, C)) return rc;
-    if (int rc = put(&c->d_S, S)) return rc;
-    if (int rc = put(&c->d_hb, hb)) return rc;
-    if (int rc = put(&c->d_rb, rb)) return rc;
-    if (int rc = put(&c->d_lb, lb)) return rc;
+    pack(0, W);
+    pack(nwt, C);
+    pack(2 * nwt, S);
+    pack(3 * nwt, hb);
+    pack(3 * nwt + na, rb);
+    pack(3 * nwt + na + N, lb);
+    HIPCK(hipMemcpyAsync(c->d_W, c->staging.data(), total * rs, hipMemcpyHostToDevice, c->stream));
     c->moments_valid = false;
     return 0;
 }
@@ -704,6 +728,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     c->lds_limit = (size_t)lds;
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCK(hipEventCreate(&e));
+    for (auto& e : c->sxev) HIPCK(hipEventCreate(&e));
     std::vector<uint8_t> perm8(c->perm.begin(), c->perm.end());
     if (upload(ctx, &c->d_perm, perm8.data(), perm8.size())) return bad(c->err);
     std::vector<int32_t> inv((size_t)c->nsymm * c->N), vbs((size_t)c->nsymm * c->nvb);
@@ -731,7 +756,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
 void rbmgpu_destroy(rbmgpu_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->d_W,      c->d_C,      c->d_S,      c->d_hb,    c->d_rb,     c->d_lb,     c->d_perm,
+    void* ptrs[] = {c->d_W,      c->d_perm,
                     c->d_invperm, c->d_vbsite, c->d_pb,     c->d_tk,    c->d_tgroup, c->d_rowbase, c->d_rowptr,
                     c->d_tsites, c->d_mask,   c->d_val,    c->d_states, c->d_obs,   c->d_T,      c->d_O,
                     c->d_acc,    c->d_prop,   c->d_part,   c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
@@ -740,6 +765,8 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->sxev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -908,22 +935,25 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
     while (true) {
         int nb = (int)std::min<int64_t>(batch, mi - launched);
         if (nb <= 0) break;
-        HIPCK(hipEventRecord(c->ev[6], c->stream));
         for (int i = 0; i < nb; i++) {
+            HIPCK(hipEventRecord(c->sxev[2 * i], c->stream));
             if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st)) return rc;
+            HIPCK(hipEventRecord(c->sxev[2 * i + 1], c->stream));
             hipLaunchKernelGGL(k_cg_step, dim3(1), dim3(1024), 0, c->stream, c->d_x, c->d_r, c->d_p, c->d_Ap,
                                (int)c->P, c->d_st);
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(c->ev[7], c->stream));
         HIPCK(hipMemcpyAsync(&st, c->d_st, sizeof(CgStatus), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
-        launched += nb;
-        int64_t done_in_batch = st.done ? std::min<int64_t>(nb, st.itn + 1 - (launched - nb)) : nb;
-        if (done_in_batch == nb) {  // whole batch did work: use it for the per-matvec time
-            sx_ms += elapsed(c, 6, 7);
-            sx_n += nb;
+        // matvecs that did work in this batch: all, or up to the converging one
+        int64_t worked = st.done ? std::min<int64_t>(nb, st.itn + 1 - launched) : nb;
+        for (int64_t i = 0; i < worked; i++) {
+            float ms = 0;
+            HIPCK(hipEventElapsedTime(&ms, c->sxev[2 * i], c->sxev[2 * i + 1]));
+            sx_ms += ms;
+            sx_n++;
         }
+        launched += nb;
         if (st.done) break;
     }
     HIPCK(hipMemcpyAsync(x, c->d_x, c->P * 16, hipMemcpyDeviceToHost, c->stream));
