@@ -67,8 +67,11 @@ typedef struct rbmgpu_config {
     /* ---- execution */
     int32_t precision;       /* RBMGPU_FP32 or RBMGPU_FP64 */
     int32_t lanes_per_chain; /* 0: autotune */
-    int32_t keep_o;          /* materialise the O matrix (needed by
-                                rbmgpu_get_samples O and the dense S.x) */
+    int32_t keep_o;          /* 1: always materialise the O matrix.  0:
+                                a basic RBM in FP32 uses the structured
+                                S.x (O never formed, rbmgpu_get_samples
+                                cannot return O); other machines
+                                materialise O regardless */
 } rbmgpu_config;
 
 typedef struct rbmgpu_moments {
@@ -100,7 +103,7 @@ int rbmgpu_update_weights(rbmgpu_ctx* ctx, const double* dw);
 /* metropolis_sampler::sample + evaluate_and_aggregate for every kept sample
  * (metropolis_sampler.cpp:55-116, abstract_sampler.cpp:78-101): runs all
  * chains of one epoch on the GPU; states, local energies, tanh(theta) and
- * (keep_o) the O matrix stay on the device. */
+ * the O matrix (unless structured, see keep_o) stay on the device. */
 int rbmgpu_sample_epoch(rbmgpu_ctx* ctx, uint32_t epoch);
 /* operate on a host-given sample set instead (parity hook): evaluates
  * local observables, theta and O for these states as if they were sampled */
@@ -131,7 +134,8 @@ int rbmgpu_eval_logpsi(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n,
 int rbmgpu_psi_ratio(rbmgpu_ctx* ctx, const uint64_t* state, const int32_t* flips,
                      int32_t n_flips, double* ratio);
 /* copies of the current sample set: states (S x n_words), local
- * observables (S x n_groups complex), O (P x S complex; needs keep_o),
+ * observables (S x n_groups complex), O (P x S complex; not in
+ * structured mode),
  * per-chain acceptance (n_chains).  Any pointer may be NULL. */
 int rbmgpu_get_samples(const rbmgpu_ctx* ctx, uint64_t* states, double* obs,
                        double* O, double* chain_acc);

@@ -76,7 +76,7 @@ template <typename R>
 struct OArgs {
     Dims d;
     int64_t S;
-    int P, P_rbm;
+    int P, P_rbm, ld;
     const uint64_t* states;
     const cplx<R>* T;
     const int32_t* vb_site;  // n_symm x n_vb: perm_s(basis_i)
@@ -87,7 +87,7 @@ struct OArgs {
 template <typename R>
 struct MomArgs {
     int64_t S, chunk;
-    int P, n_groups;
+    int P, n_groups, ld;
     const cplx<R>* O;
     const double* obs;
     double* part;
@@ -102,18 +102,38 @@ struct CgStatus {
 
 template <typename R>
 struct SxArgs {
-    int64_t S, nwaves;
-    int P, n_neural;
+    int64_t S, nblocks;
+    int P, n_neural, ld, nchunks;
     const cplx<R>* O;
     const double* x;     // complex P
     const double* mean;  // complex P
     const double* var;   // P
     const double* maxvar;
     double r1, r2, rd;
-    cplx<R>* part;       // nwaves x P
+    cplx<R>* part;       // nblocks x ld
+    double* part2;       // nchunks x P complex
     double* y;           // complex P
     const CgStatus* status;
 };
+
+// structured (basic RBM) S.x / moments, FP32 MFMA; see kern_struct.hpp
+struct StructArgs {
+    int64_t S;
+    int N, M, P, ld, nw, n_groups, mode;  // mode 0: S.x, 1: y = E_loc, 2: y = 1
+    const uint64_t* states;
+    const cplx<float>* T;
+    const double* obs;
+    const double* x;
+    cplx<float>* part;  // nblocks x ld: [z_v | z_h | Z | sum |t|^2]
+    const CgStatus* status;
+};
+// LDS bytes of k_struct for padded sizes, 0 if no instance covers N, M
+size_t struct_lds_bytes(int N, int M);
+hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st);
+template <typename R>
+hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st);
+template <typename R>
+hipError_t launch_sx_finish(const SxArgs<R>& a, hipStream_t st);
 
 // supported (lanes per chain, units per lane) shapes
 struct Shape {

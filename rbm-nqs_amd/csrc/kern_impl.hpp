@@ -369,8 +369,9 @@ __global__ void __launch_bounds__(256) k_build_O(OArgs<R> a) {
         sg[f] = ((a.states[k * d.nw + (f >> 6)] >> (f & 63)) & 1) ? -1.f : 1.f;
     for (int h = threadIdx.x; h < d.H; h += blockDim.x) tr[h] = a.T[k * d.H + h];
     __syncthreads();
-    cplx<R>* o = a.O + k * a.P;
+    cplx<R>* o = a.O + k * a.ld;
     int nvb = d.n_vb, na = d.n_alpha;
+    for (int p = a.P + threadIdx.x; p < a.ld; p += blockDim.x) o[p] = {0, 0};
     for (int p = threadIdx.x; p < a.P_rbm; p += blockDim.x) {
         cplx<R> v{0, 0};
         if (p < nvb) {
@@ -402,7 +403,7 @@ __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
     for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
         double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0;
         for (int64_t k = k0; k < k1; k++) {
-            cplx<R> o = a.O[k * a.P + p];
+            cplx<R> o = a.O[k * a.ld + p];
             double ore = o.re, oim = o.im;
             double ere = a.obs[k * 2 * a.n_groups], eim = a.obs[k * 2 * a.n_groups + 1];
             s0 += ore;
@@ -421,43 +422,120 @@ __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
 }
 
 // ------------------------------------------------------------------ S.x --
-// one pass over O: every wavefront streams whole rows k (samples), forms
-// y_k = O_k . b with a wave reduction and accumulates conj(O_k) y_k into
-// lane-strided registers; per-wave partials are summed by k_sx_finish
-template <typename R, int ZM>
+// One pass over O (rows of ld complex, ld = P rounded up to even, so every
+// pair of entries is one 16-byte load in FP32).  Each wavefront streams
+// whole rows k: y_k = O_k . x by a wave reduction, then conj(O_k) y_k is
+// accumulated in lane registers (FP32: the row stays in registers; FP64:
+// the row is re-read from cache).  The four waves of a block add their
+// partials in LDS; the block partials are reduced by k_sx_reduce and
+// finished by k_sx_finish (regularisation and the <O> term).
+template <typename R>
+__device__ inline void load_pair(const cplx<R>* o, int p, cplx<R>& a, cplx<R>& b);
+template <>
+__device__ inline void load_pair<float>(const cplx<float>* o, int p, cplx<float>& a, cplx<float>& b) {
+    float4 v = *reinterpret_cast<const float4*>(o + p);
+    a = {v.x, v.y};
+    b = {v.z, v.w};
+}
+template <>
+__device__ inline void load_pair<double>(const cplx<double>* o, int p, cplx<double>& a, cplx<double>& b) {
+    double2 v0 = *reinterpret_cast<const double2*>(o + p);
+    double2 v1 = *reinterpret_cast<const double2*>(o + p + 1);
+    a = {v0.x, v0.y};
+    b = {v1.x, v1.y};
+}
+
+template <typename R, int ZM>  // ZM pairs per lane: ld <= 128 * ZM
 __global__ void __launch_bounds__(256) k_sx_partial(SxArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     if (a.status && a.status->done) return;
     cplx<R>* b = reinterpret_cast<cplx<R>*>(smem);
-    for (int p = threadIdx.x; p < a.P; p += blockDim.x) b[p] = {(R)a.x[2 * p], (R)a.x[2 * p + 1]};
+    cplx<R>* zs = b + a.ld;
+    for (int p = threadIdx.x; p < a.ld; p += blockDim.x) {
+        b[p] = p < a.P ? cplx<R>{(R)a.x[2 * p], (R)a.x[2 * p + 1]} : cplx<R>{0, 0};
+        zs[p] = {0, 0};
+    }
     __syncthreads();
-    int lane = threadIdx.x & 63;
-    int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    cplx<R> z[ZM];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    cplx<R> z[2 * ZM];
 #pragma unroll
-    for (int i = 0; i < ZM; i++) z[i] = {0, 0};
-    for (int64_t k = wave; k < a.S; k += nwaves) {
-        const cplx<R>* o = a.O + k * a.P;
+    for (int i = 0; i < 2 * ZM; i++) z[i] = {0, 0};
+    for (int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv; k < a.S; k += nwaves) {
+        const cplx<R>* o = a.O + k * a.ld;
         cplx<R> dot{0, 0};
+        if constexpr (sizeof(R) == 4) {
+            cplx<R> v[2 * ZM];
 #pragma unroll
-        for (int i = 0; i < ZM; i++) {
-            int p = lane + 64 * i;
-            if (p < a.P) dot = dot + o[p] * b[p];
-        }
-        dot = group_csum<64>(dot);
+            for (int i = 0; i < ZM; i++) {
+                int p = 2 * lane + 128 * i;
+                if (p < a.ld) load_pair<R>(o, p, v[2 * i], v[2 * i + 1]);
+                else v[2 * i] = v[2 * i + 1] = {0, 0};
+            }
 #pragma unroll
-        for (int i = 0; i < ZM; i++) {
-            int p = lane + 64 * i;
-            if (p < a.P) z[i] = z[i] + conj(o[p]) * dot;
+            for (int i = 0; i < ZM; i++) {
+                int p = 2 * lane + 128 * i;
+                if (p < a.ld) dot = dot + v[2 * i] * b[p] + v[2 * i + 1] * b[p + 1];
+            }
+            dot = group_csum<64>(dot);
+#pragma unroll
+            for (int i = 0; i < 2 * ZM; i++) z[i] = z[i] + conj(v[i]) * dot;
+        } else {
+#pragma unroll
+            for (int i = 0; i < ZM; i++) {
+                int p = 2 * lane + 128 * i;
+                if (p < a.ld) {
+                    cplx<R> v0, v1;
+                    load_pair<R>(o, p, v0, v1);
+                    dot = dot + v0 * b[p] + v1 * b[p + 1];
+                }
+            }
+            dot = group_csum<64>(dot);
+#pragma unroll
+            for (int i = 0; i < ZM; i++) {
+                int p = 2 * lane + 128 * i;
+                if (p < a.ld) {
+                    cplx<R> v0, v1;
+                    load_pair<R>(o, p, v0, v1);
+                    z[2 * i] = z[2 * i] + conj(v0) * dot;
+                    z[2 * i + 1] = z[2 * i + 1] + conj(v1) * dot;
+                }
+            }
         }
     }
-    cplx<R>* out = a.part + wave * a.P;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        if (wv == w) {
 #pragma unroll
-    for (int i = 0; i < ZM; i++) {
-        int p = lane + 64 * i;
-        if (p < a.P) out[p] = z[i];
+            for (int i = 0; i < ZM; i++) {
+                int p = 2 * lane + 128 * i;
+                if (p < a.ld) {
+                    zs[p] = zs[p] + z[2 * i];
+                    zs[p + 1] = zs[p + 1] + z[2 * i + 1];
+                }
+            }
+        }
+        __syncthreads();
     }
+    cplx<R>* out = a.part + (int64_t)blockIdx.x * a.ld;
+    for (int p = threadIdx.x; p < a.ld; p += blockDim.x) out[p] = zs[p];
+}
+
+// sums the block partials of one chunk of blocks, in FP64
+template <typename R>
+__global__ void __launch_bounds__(256) k_sx_reduce(SxArgs<R> a) {
+    if (a.status && a.status->done) return;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    int64_t per = (a.nblocks + gridDim.y - 1) / gridDim.y;
+    int64_t b0 = blockIdx.y * per, b1 = b0 + per < a.nblocks ? b0 + per : a.nblocks;
+    double zr = 0, zi = 0;
+    for (int64_t b = b0; b < b1; b++) {
+        cplx<R> v = a.part[b * a.ld + p];
+        zr += v.re;
+        zi += v.im;
+    }
+    a.part2[2 * (blockIdx.y * (int64_t)a.P + p)] = zr;
+    a.part2[2 * (blockIdx.y * (int64_t)a.P + p) + 1] = zi;
 }
 
 template <typename R>
@@ -486,10 +564,9 @@ __global__ void __launch_bounds__(256) k_sx_finish(SxArgs<R> a) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.P) return;
     double zr = 0, zi = 0;
-    for (int64_t w = 0; w < a.nwaves; w++) {
-        cplx<R> v = a.part[w * a.P + p];
-        zr += v.re;
-        zi += v.im;
+    for (int c = 0; c < a.nchunks; c++) {
+        zr += a.part2[2 * ((int64_t)c * a.P + p)];
+        zi += a.part2[2 * ((int64_t)c * a.P + p) + 1];
     }
     double inv = 1.0 / (double)a.S;
     double reg = a.var[p] * (p < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;

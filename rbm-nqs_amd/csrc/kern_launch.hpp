@@ -67,26 +67,47 @@ hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st) {
 
 template <typename R, int ZM>
 static hipError_t sx_one(const SxArgs<R>& a, int blocks, hipStream_t st) {
-    size_t lds = (size_t)a.P * sizeof(cplx<R>);
+    size_t lds = 2 * (size_t)a.ld * sizeof(cplx<R>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sx_partial<R, ZM>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), lds, st, a);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_sx_reduce<R>), dim3((a.P + 255) / 256, a.nchunks), dim3(256), 0, st, a);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_sx_finish<R>), dim3((a.P + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st) {
-    if (a.P <= 64 * 2) return sx_one<R, 2>(a, blocks, st);
-    if (a.P <= 64 * 4) return sx_one<R, 4>(a, blocks, st);
-    if (a.P <= 64 * 8) return sx_one<R, 8>(a, blocks, st);
-    if (a.P <= 64 * 16) return sx_one<R, 16>(a, blocks, st);
-    if (a.P <= 64 * 24) return sx_one<R, 24>(a, blocks, st);
-    if (a.P <= 64 * 36) return sx_one<R, 36>(a, blocks, st);
-    if (a.P <= 64 * 48) return sx_one<R, 48>(a, blocks, st);
+    if (a.ld <= 128 * 2) return sx_one<R, 2>(a, blocks, st);
+    if (a.ld <= 128 * 4) return sx_one<R, 4>(a, blocks, st);
+    if (a.ld <= 128 * 8) return sx_one<R, 8>(a, blocks, st);
+    if (a.ld <= 128 * 12) return sx_one<R, 12>(a, blocks, st);
+    if (a.ld <= 128 * 18) return sx_one<R, 18>(a, blocks, st);
+    if (a.ld <= 128 * 24) return sx_one<R, 24>(a, blocks, st);
     return hipErrorInvalidValue;
 }
 
+template <typename R>
+hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_sx_reduce<R>), dim3((a.P + 255) / 256, a.nchunks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_sx_finish(const SxArgs<R>& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_sx_finish<R>), dim3((a.P + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 #define RBM_INSTANTIATE(R)                                                                  \
+    template hipError_t launch_sx_reduce<R>(const SxArgs<R>&, hipStream_t);               \
+    template hipError_t launch_sx_finish<R>(const SxArgs<R>&, hipStream_t);               \
     template hipError_t launch_sample<R>(const SampArgs<R>&, Shape, int, hipStream_t);    \
     template hipError_t launch_eval<R>(const EvalArgs<R>&, Shape, int, hipStream_t);      \
     template hipError_t launch_build_O<R>(const OArgs<R>&, hipStream_t);                  \

@@ -1,0 +1,56 @@
+// FP32 MFMA instances of the structured (basic RBM) S.x / moments kernel.
+#include "kern_struct.hpp"
+
+namespace rbmgpu {
+
+#define STRUCT_SHAPES(X) \
+    X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4) X(8, 1) X(8, 2)
+
+static bool pick(int N, int M, int& ct, int& rt) {
+    ct = rt = 0;
+    int best = 1 << 30;
+#define X(C_, R_)                                                              \
+    if (16 * C_ >= M && 16 * R_ >= N && C_ * R_ * 256 < best) {              \
+        best = C_ * R_ * 256;                                                  \
+        ct = C_;                                                               \
+        rt = R_;                                                               \
+    }
+    STRUCT_SHAPES(X)
+#undef X
+    return ct != 0;
+}
+
+static size_t lds_for(int ct, int rt) {
+    size_t Mp = 16 * ct, Np = 16 * rt;
+    return 2 * Np * Mp * 4 + (size_t)kTS * Np * 4 + (size_t)kTS * Mp * 8 + kTS * 8 + 2 * Np * 4 + 2 * Mp * 4;
+}
+
+size_t struct_lds_bytes(int N, int M) {
+    int ct, rt;
+    if (!pick(N, M, ct, rt)) return 0;
+    return lds_for(ct, rt);
+}
+
+template <int CT, int RT>
+static hipError_t one(const StructArgs& a, int blocks, hipStream_t st) {
+    size_t lds = lds_for(CT, RT);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_struct<CT, RT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_struct<CT, RT>), dim3(blocks), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st) {
+    int ct, rt;
+    if (!pick(a.N, a.M, ct, rt)) return hipErrorInvalidValue;
+#define X(C_, R_) \
+    if (ct == C_ && rt == R_) return one<C_, R_>(a, blocks, st);
+    STRUCT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+}  // namespace rbmgpu

@@ -18,12 +18,12 @@
 using namespace rbmgpu;
 using zd = std::complex<double>;
 
-int rbmgpu::sx_max_params() { return 64 * 48; }
+int rbmgpu::sx_max_params() { return 128 * 24; }
 
 struct rbmgpu_ctx {
     // ---- model
     int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0;
-    int64_t P = 0, P_rbm = 0;
+    int64_t P = 0, P_rbm = 0, ld = 0;  // ld: O row length (P rounded up to even)
     std::vector<int32_t> perm, basis, pb;
     int n_terms = 0, n_groups = 1;
     // ---- sampler
@@ -51,6 +51,9 @@ struct rbmgpu_ctx {
     void* d_O = nullptr;
     uint32_t *d_acc = nullptr, *d_prop = nullptr;
     bool have_chains = false;
+    // structured: basic RBM in FP32 without an O matrix (kern_struct.hpp)
+    bool structured = false;
+    int64_t ld_s = 0;  // structured partial row: [P | sum |t|^2 (M)], even
     // moments
     double *d_part = nullptr, *d_omean = nullptr, *d_ovar = nullptr, *d_eo = nullptr,
            *d_maxvar = nullptr, *d_emom = nullptr;
@@ -60,7 +63,9 @@ struct rbmgpu_ctx {
     double *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr;
     CgStatus* d_st = nullptr;
     void* d_sxpart = nullptr;
-    int64_t sx_waves = 0, sxpart_cap = 0;
+    double* d_sxpart2 = nullptr;
+    int64_t sx_blocks = 0, sxpart_cap = 0;
+    static constexpr int kSxChunks = 32;
     // eval scratch
     uint64_t* d_evst = nullptr;
     double *d_evout = nullptr, *d_evobs = nullptr;
@@ -188,8 +193,8 @@ int ensure_samples(rbmgpu_ctx* c, int64_t S) {
     if (int rc = dalloc(ctx, &c->d_states, S * c->nw * sizeof(uint64_t))) return rc;
     if (int rc = dalloc(ctx, &c->d_obs, S * 2 * c->n_groups * sizeof(double))) return rc;
     if (int rc = dalloc(ctx, &c->d_T, S * c->H * 2 * rsize(c))) return rc;
-    if (c->keep_o)
-        if (int rc = dalloc(ctx, &c->d_O, S * c->P * 2 * rsize(c))) return rc;
+    if (!c->structured)
+        if (int rc = dalloc(ctx, &c->d_O, S * c->ld * 2 * rsize(c))) return rc;
     c->cap = S;
     return 0;
 }
@@ -233,12 +238,13 @@ int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samp
 template <typename R>
 int run_build_O(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
-    if (!c->keep_o || c->S == 0) return 0;
+    if (c->structured || c->S == 0) return 0;
     OArgs<R> a{};
     a.d = dims(c);
     a.S = c->S;
     a.P = (int)c->P;
     a.P_rbm = (int)c->P_rbm;
+    a.ld = (int)c->ld;
     a.states = c->d_states;
     a.T = (const cplx<R>*)c->d_T;
     a.vb_site = c->d_vbsite;
@@ -475,6 +481,79 @@ __global__ void __launch_bounds__(1024) k_cg_step(double* x, double* r, double* 
     }
 }
 
+// <O>, Var(O), <E O*> of the basic RBM from the structured sums:
+// A = sum conj(O) and sum |t_i|^2 (mode 2), B = sum E conj(O) (mode 1),
+// each as 32 chunk partials
+__global__ void __launch_bounds__(256) k_struct_moments(const double* A, const double* B, int nch, int64_t S,
+                                                        int N, int M, int P, double* mean, double* var,
+                                                        double* eo, unsigned long long* maxvar) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int PA = P + M;
+    auto sumA = [&](int q, double& re, double& im) {
+        re = im = 0;
+        for (int c = 0; c < nch; c++) {
+            re += A[2 * ((int64_t)c * PA + q)];
+            im += A[2 * ((int64_t)c * PA + q) + 1];
+        }
+    };
+    double ar, ai, br = 0, bi = 0, t2, dummy;
+    sumA(p, ar, ai);
+    for (int c = 0; c < nch; c++) {
+        br += B[2 * ((int64_t)c * P + p)];
+        bi += B[2 * ((int64_t)c * P + p) + 1];
+    }
+    double inv = 1.0 / (double)S;
+    double mr = ar * inv, mi = -ai * inv;  // mean = conj(sum conj(O)) / S
+    double so2;
+    if (p < N) {
+        so2 = (double)S;  // sigma^2 = 1
+    } else {
+        int i = p < N + M ? p - N : (p - N - M) / N;
+        sumA(P + i, t2, dummy);
+        so2 = t2;
+    }
+    double v = fmax((so2 - (double)S * (mr * mr + mi * mi)) / ((double)S - 1.0), 0.0);
+    mean[2 * p] = mr;
+    mean[2 * p + 1] = mi;
+    var[p] = v;
+    eo[2 * p] = br * inv;
+    eo[2 * p + 1] = bi * inv;
+    atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
+}
+
+int ensure_sx(rbmgpu_ctx* c);
+int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st);
+
+int struct_moments(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (int rc = ensure_sx(c)) return rc;
+    const int nch = rbmgpu_ctx::kSxChunks;
+    double* A = c->d_sxpart2;
+    double* B = A + (int64_t)nch * (c->P + c->na) * 2;
+    SxArgs<float> a{};
+    a.nblocks = c->sx_blocks;
+    a.ld = (int)c->ld_s;
+    a.nchunks = nch;
+    a.part = (cplx<float>*)c->d_sxpart;
+    // mode 2: sum conj(O) and sum |t|^2
+    if (int rc = run_struct(c, 2, nullptr, nullptr)) return rc;
+    a.P = (int)(c->P + c->na);
+    a.part2 = A;
+    HIPCK(launch_sx_reduce<float>(a, c->stream));
+    // mode 1: sum E conj(O)
+    if (int rc = run_struct(c, 1, nullptr, nullptr)) return rc;
+    a.P = (int)c->P;
+    a.part2 = B;
+    HIPCK(launch_sx_reduce<float>(a, c->stream));
+    HIPCK(hipMemsetAsync(c->d_maxvar, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_struct_moments, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, A, B, nch,
+                       c->S, c->N, c->na, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
+                       (unsigned long long*)c->d_maxvar);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
 int ensure_cg(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
     if (c->d_x) return 0;
@@ -490,15 +569,28 @@ int ensure_cg(rbmgpu_ctx* c) {
 
 int ensure_sx(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
-    if (c->P > sx_max_params()) return fail(c, "too many parameters for the S.x kernel");
-    int64_t w = std::min<int64_t>(std::max<int64_t>(c->S / 32, 4), 4096);
-    w = (w + 3) / 4 * 4;
-    c->sx_waves = w;
-    int64_t need = w * c->P * 2 * rsize(c);
+    int64_t blocks, row;
+    if (c->structured) {
+        // tiles of 64 samples, a few tiles per block
+        int64_t tiles = (c->S + 63) / 64;
+        blocks = std::min<int64_t>(tiles, 4 * (int64_t)c->n_cu);
+        row = c->ld_s;
+    } else {
+        if (c->ld > sx_max_params()) return fail(c, "too many parameters for the S.x kernel");
+        // about 4 blocks of 4 waves per CU, each wave streaming >= 8 rows
+        blocks = std::min<int64_t>(std::max<int64_t>(c->S / 32, 1), 4 * (int64_t)c->n_cu);
+        row = c->ld;
+    }
+    c->sx_blocks = blocks;
+    int64_t need = blocks * row * 2 * rsize(c);
     if (need > c->sxpart_cap) {
         if (int rc = dalloc(ctx, &c->d_sxpart, need)) return rc;
         c->sxpart_cap = need;
     }
+    if (!c->d_sxpart2)
+        if (int rc = dalloc(ctx, &c->d_sxpart2,
+                            rbmgpu_ctx::kSxChunks * (2 * c->P + c->na) * 2 * sizeof(double)))
+            return rc;
     return 0;
 }
 
@@ -507,8 +599,11 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
     rbmgpu_ctx* ctx = c;
     SxArgs<R> a{};
     a.S = c->S;
-    a.nwaves = c->sx_waves;
+    a.nblocks = c->sx_blocks;
     a.P = (int)c->P;
+    a.ld = (int)c->ld;
+    a.nchunks = rbmgpu_ctx::kSxChunks;
+    a.part2 = c->d_sxpart2;
     a.n_neural = (int)c->P_rbm;
     a.O = (const cplx<R>*)c->d_O;
     a.x = x;
@@ -521,11 +616,57 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
     a.part = (cplx<R>*)c->d_sxpart;
     a.y = y;
     a.status = st;
-    HIPCK(launch_sx<R>(a, (int)(c->sx_waves / 4), c->stream));
+    HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream));
+    return 0;
+}
+
+int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st) {
+    rbmgpu_ctx* ctx = c;
+    StructArgs a{};
+    a.S = c->S;
+    a.N = c->N;
+    a.M = c->na;
+    a.P = (int)c->P;
+    a.ld = (int)c->ld_s;
+    a.nw = c->nw;
+    a.n_groups = c->n_groups;
+    a.mode = mode;
+    a.states = c->d_states;
+    a.T = (const cplx<float>*)c->d_T;
+    a.obs = c->d_obs;
+    a.x = x;
+    a.part = (cplx<float>*)c->d_sxpart;
+    a.status = st;
+    HIPCK(launch_struct(a, (int)c->sx_blocks, c->stream));
     return 0;
 }
 
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st) {
+    if (c->structured) {
+        rbmgpu_ctx* ctx = c;
+        if (int rc = run_struct(c, 0, x, st)) return rc;
+        SxArgs<float> a{};
+        a.S = c->S;
+        a.nblocks = c->sx_blocks;
+        a.P = (int)c->P;
+        a.ld = (int)c->ld_s;
+        a.nchunks = rbmgpu_ctx::kSxChunks;
+        a.part = (cplx<float>*)c->d_sxpart;
+        a.part2 = c->d_sxpart2;
+        a.n_neural = (int)c->P_rbm;
+        a.x = x;
+        a.mean = c->d_omean;
+        a.var = c->d_ovar;
+        a.maxvar = c->d_maxvar;
+        a.r1 = r1;
+        a.r2 = r2;
+        a.rd = rd;
+        a.y = y;
+        a.status = st;
+        HIPCK(launch_sx_reduce<float>(a, c->stream));
+        HIPCK(launch_sx_finish<float>(a, c->stream));
+        return 0;
+    }
     return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st)
                                   : run_sx<double>(c, x, y, r1, r2, rd, st);
 }
@@ -546,7 +687,9 @@ int compute_moments(rbmgpu_ctx* c) {
     hipLaunchKernelGGL(k_e_moments, dim3(1), dim3(1024), 0, c->stream, c->d_obs, c->S, c->n_groups, nb,
                        c->d_emom);
     HIPCK(hipGetLastError());
-    if (c->keep_o) {
+    if (c->structured) {
+        if (int rc = struct_moments(c)) return rc;
+    } else {
         int64_t chunk = 256;
         int64_t blocks = (c->S + chunk - 1) / chunk;
         if (blocks > 4096) {
@@ -560,10 +703,11 @@ int compute_moments(rbmgpu_ctx* c) {
             c->part_cap = need;
         }
         if (c->prec == RBMGPU_FP32) {
-            MomArgs<float> a{c->S, chunk, (int)c->P, c->n_groups, (const cplx<float>*)c->d_O, c->d_obs, c->d_part};
+            MomArgs<float> a{c->S, chunk, (int)c->P, c->n_groups, (int)c->ld, (const cplx<float>*)c->d_O, c->d_obs,
+                             c->d_part};
             HIPCK(launch_o_partial<float>(a, (int)blocks, c->stream));
         } else {
-            MomArgs<double> a{c->S, chunk, (int)c->P, c->n_groups, (const cplx<double>*)c->d_O, c->d_obs,
+            MomArgs<double> a{c->S, chunk, (int)c->P, c->n_groups, (int)c->ld, (const cplx<double>*)c->d_O, c->d_obs,
                               c->d_part};
             HIPCK(launch_o_partial<double>(a, (int)blocks, c->stream));
         }
@@ -652,6 +796,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     c->pf_rows = cfg->pf_rows;
     if (c->pf_rows != 0) return bad("Pfaffian factor: not supported by this build of the library");
     c->P = c->P_rbm;
+    c->ld = c->P + (c->P & 1);
     if (!cfg->perm || !cfg->basis) return bad("perm and basis tables are required");
     c->perm.assign(cfg->perm, cfg->perm + (int64_t)c->nsymm * c->N);
     c->basis.assign(cfg->basis, cfg->basis + c->nvb);
@@ -747,6 +892,12 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (upload(ctx, &c->d_mask, mask.data(), mask.size())) return bad(c->err);
     if (upload(ctx, &c->d_val, val.data(), val.size() * 8)) return bad(c->err);
     if (dalloc(ctx, &c->d_acc, c->n_chains * 4) || dalloc(ctx, &c->d_prop, c->n_chains * 4)) return bad(c->err);
+    {
+        size_t sl = struct_lds_bytes(c->N, c->na);
+        c->structured = !c->keep_o && c->nsymm == 1 && c->nvb == c->N && c->pf_rows == 0 &&
+                        c->prec == RBMGPU_FP32 && sl > 0 && sl <= c->lds_limit;
+        c->ld_s = c->P + c->na + ((c->P + c->na) & 1);
+    }
     c->params.assign(c->P, 0.);
     if (build_weight_tables(c)) return bad(c->err);
     *out = c;
@@ -761,7 +912,7 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_tsites, c->d_mask,   c->d_val,    c->d_states, c->d_obs,   c->d_T,      c->d_O,
                     c->d_acc,    c->d_prop,   c->d_part,   c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
                     c->d_emom,   c->d_x,      c->d_b,      c->d_r,     c->d_p,      c->d_Ap,     c->d_st,
-                    c->d_sxpart, c->d_evst,   c->d_evout,  c->d_evobs};
+                    c->d_sxpart, c->d_sxpart2, c->d_evst,   c->d_evout,  c->d_evobs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev)
@@ -861,13 +1012,9 @@ int rbmgpu_reduce(rbmgpu_ctx* c, rbmgpu_moments* m, double* o_mean, double* o_va
         HIPCK(hipMemcpyAsync(acc.data(), c->d_acc, c->n_chains * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipMemcpyAsync(prop.data(), c->d_prop, c->n_chains * 4, hipMemcpyDeviceToHost, c->stream));
     }
-    if (c->keep_o) {
-        if (o_mean) HIPCK(hipMemcpyAsync(o_mean, c->d_omean, c->P * 16, hipMemcpyDeviceToHost, c->stream));
-        if (o_var) HIPCK(hipMemcpyAsync(o_var, c->d_ovar, c->P * 8, hipMemcpyDeviceToHost, c->stream));
-        if (eo) HIPCK(hipMemcpyAsync(eo, c->d_eo, c->P * 16, hipMemcpyDeviceToHost, c->stream));
-    } else if (o_mean || o_var || eo) {
-        return fail(c, "O statistics need keep_o");
-    }
+    if (o_mean) HIPCK(hipMemcpyAsync(o_mean, c->d_omean, c->P * 16, hipMemcpyDeviceToHost, c->stream));
+    if (o_var) HIPCK(hipMemcpyAsync(o_var, c->d_ovar, c->P * 8, hipMemcpyDeviceToHost, c->stream));
+    if (eo) HIPCK(hipMemcpyAsync(eo, c->d_eo, c->P * 16, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     if (m) {
         m->e_mean[0] = e[0];
@@ -893,7 +1040,6 @@ int rbmgpu_reduce(rbmgpu_ctx* c, rbmgpu_moments* m, double* o_mean, double* o_va
 int rbmgpu_apply_S(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd) {
     if (!c || !x || !y) return -1;
     rbmgpu_ctx* ctx = c;
-    if (!c->keep_o) return fail(c, "S.x needs keep_o");
     if (!c->moments_valid)
         if (int rc = compute_moments(c)) return rc;
     if (int rc = ensure_cg(c)) return rc;
@@ -912,7 +1058,6 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
                     int64_t max_iter, double rtol, int64_t* itn, double* rs) {
     if (!c || !b || !x) return -1;
     rbmgpu_ctx* ctx = c;
-    if (!c->keep_o) return fail(c, "S.x needs keep_o");
     if (!c->moments_valid)
         if (int rc = compute_moments(c)) return rc;
     if (int rc = ensure_cg(c)) return rc;
@@ -1001,13 +1146,14 @@ int rbmgpu_get_samples(const rbmgpu_ctx* cc, uint64_t* states, double* obs, doub
     if (states) HIPCK(hipMemcpy(states, c->d_states, c->S * c->nw * 8, hipMemcpyDeviceToHost));
     if (obs) HIPCK(hipMemcpy(obs, c->d_obs, c->S * 2 * c->n_groups * 8, hipMemcpyDeviceToHost));
     if (O) {
-        if (!c->keep_o) return fail(c, "O not kept");
+        if (c->structured) return fail(c, "the O matrix is not materialised in structured mode (keep_o = 0)");
         if (c->prec == RBMGPU_FP64) {
-            HIPCK(hipMemcpy(O, c->d_O, c->S * c->P * 16, hipMemcpyDeviceToHost));
+            HIPCK(hipMemcpy2D(O, c->P * 16, c->d_O, c->ld * 16, c->P * 16, c->S, hipMemcpyDeviceToHost));
         } else {
-            std::vector<float> f(c->S * c->P * 2);
+            std::vector<float> f(c->S * c->ld * 2);
             HIPCK(hipMemcpy(f.data(), c->d_O, f.size() * 4, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < f.size(); i++) O[i] = f[i];
+            for (int64_t k = 0; k < c->S; k++)
+                for (int64_t i = 0; i < 2 * c->P; i++) O[k * 2 * c->P + i] = f[k * 2 * c->ld + i];
         }
     }
     if (chain_acc) {

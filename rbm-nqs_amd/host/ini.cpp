@@ -145,6 +145,7 @@ const std::map<std::string, setter>& table() {
              c.gpu_precision = p;
          }},
         {"gpu.lanes_per_chain", [](config& c, const std::string& v) { c.gpu_lanes = std::stoi(v); }},
+        {"gpu.keep_o", [](config& c, const std::string& v) { c.gpu_keep_o = boolean(v); }},
     };
     return t;
 }

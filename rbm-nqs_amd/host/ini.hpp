@@ -72,6 +72,8 @@ struct config {
     // GPU backend (new keys; the reference has no GPU)
     int gpu_precision = 32;
     int gpu_lanes = 0;
+    // materialise the O matrix even where the structured S.x applies
+    bool gpu_keep_o = false;
 };
 
 // Parse .ini text into cfg (unknown keys are an error, like

@@ -148,7 +148,7 @@ rbmgpu_config setup::gpu_config() const {
     c.seed = cfg_.seed;
     c.precision = cfg_.gpu_precision;
     c.lanes_per_chain = cfg_.gpu_lanes;
-    c.keep_o = 1;
+    c.keep_o = cfg_.gpu_keep_o ? 1 : 0;
     return c;
 }
 

@@ -30,10 +30,13 @@ def _wrap(x):
     return (x + np.pi) % (2 * np.pi) - np.pi
 
 
-def _setup(spec, prec, std=0.3, seed=1):
-    print(f"[gpu-test] spec={spec.split()[3:6]} prec={prec} std={std}", flush=True)
+KEEP_O = "[gpu]\nkeep_o = true\n"
+
+
+def _setup(spec, prec, std=0.3, seed=1, keep_o=False):
+    print(f"[gpu-test] spec={spec.split()[3:6]} prec={prec} std={std} keep_o={keep_o}", flush=True)
     rng = np.random.default_rng(seed)
-    m = rbmgpu.Model(spec)
+    m = rbmgpu.Model(spec + (KEEP_O if keep_o else ""))
     o = po.Oracle(spec)
     p = random_params(rng, m.P, std)
     o.set_params(p)
@@ -71,7 +74,7 @@ def test_psi_ratio(name, prec):
 @pytest.mark.parametrize("name", list(SPECS))
 @pytest.mark.parametrize("prec", [64, 32])
 def test_eloc_and_O(name, prec):
-    m, o, g, rng = _setup(SPECS[name], prec, std=0.3 if prec == 64 else 0.1)
+    m, o, g, rng = _setup(SPECS[name], prec, std=0.3 if prec == 64 else 0.1, keep_o=True)
     states = po.random_states(rng, 96, m.N)
     g.load_samples(states)
     st, obs, O, _ = g.get_samples(states=True, obs=True, O=True)
@@ -85,10 +88,24 @@ def test_eloc_and_O(name, prec):
     assert np.max(np.abs(O - O_ref)) <= tol * oscale
 
 
-@pytest.mark.parametrize("name", ["C1", "C2", "C3"])
-@pytest.mark.parametrize("prec", [64, 32])
-def test_reduce_S_and_cg(name, prec):
-    m, o, g, rng = _setup(SPECS[name], prec, std=0.2 if prec == 64 else 0.1)
+def test_structured_mode_has_no_O():
+    """A basic RBM in FP32 runs the structured S.x: the O matrix is never
+    formed, so asking for it is an error (rbmgpu.h keep_o)."""
+    m, o, g, rng = _setup(C1, 32, std=0.1)
+    g.load_samples(po.random_states(rng, 16, m.N))
+    with pytest.raises(RuntimeError, match="structured"):
+        g.get_samples(O=True)
+
+
+# (spec, precision, keep_o): FP32 basic RBMs run the structured S.x unless
+# keep_o; symmetric RBMs and FP64 always materialise O
+REDUCE_CASES = [("C1", 64, False), ("C1", 32, False), ("C1", 32, True), ("C2", 64, False),
+                ("C2", 32, False), ("C2", 32, True), ("C3", 64, False), ("C3", 32, False)]
+
+
+@pytest.mark.parametrize("name,prec,keep_o", REDUCE_CASES)
+def test_reduce_S_and_cg(name, prec, keep_o):
+    m, o, g, rng = _setup(SPECS[name], prec, std=0.2 if prec == 64 else 0.1, keep_o=keep_o)
     S = 200
     states = po.random_states(rng, S, m.N)
     g.load_samples(states)
@@ -109,7 +126,8 @@ def test_reduce_S_and_cg(name, prec):
     y = g.apply_S(x, r1, r2, rd)
     y_ref = po.apply_S(O, ref["o_mean"], ref["o_var"], r1, r2, rd, m.n_neural, x)
     # FP32: O entries carry ~1e-6 relative error, amplified by cancellation
-    # in the P-term products O_k . x -> stated tolerance 5e-4 of max |y|
+    # in the P-term products O_k . x (materialised) or the N- and M-term
+    # sums of the structured form -> stated tolerance 5e-4 of max |y|
     stol = tol if prec == 64 else 5e-4
     assert np.max(np.abs(y - y_ref)) <= stol * np.abs(y_ref).max()
     # CG with a fixed iteration count and warm start (cg_solver.cpp:119-189);
