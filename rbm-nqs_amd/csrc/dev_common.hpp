@@ -81,8 +81,30 @@ struct M<float> {
 // cancellation near the poles x = 0, y = pi/2 + k pi) and nothing overflows.
 // th and sech^2 come from em = expm1(-2|x|): th = -em / (2 + em),
 // sech^2 = 4 (1 + em) / (2 + em)^2.
+//
+// FP32 evaluates the same formula from hardware transcendentals: expm1 by
+// its Taylor polynomial for |2x| < 1/4 (truncation < 5e-8 relative) and
+// exp - 1 beyond, sin/cos of the argument reduced to [-1/2, 1/2]
+// revolutions (v_sin_f32 / v_cos_f32), reciprocals by v_rcp_f32.  The
+// result carries a few 1e-7 relative error, like every FP32 quantity of
+// the sampler.
+__device__ inline cplx<float> ctanh_fast(cplx<float> z) {
+    float ax = __builtin_fabsf(z.re), t = -2.f * ax;
+    float ps = t * (1.f + t * (0.5f + t * (1.f / 6 + t * (1.f / 24 + t * (1.f / 120 + t * (1.f / 720))))));
+    float em = ax < 0.125f ? ps : __expf(t) - 1.f;
+    float q = __builtin_amdgcn_rcpf(2.f + em);
+    float th = __builtin_copysignf(-em * q, z.re);
+    float sech2 = 4.f * (1.f + em) * q * q;
+    float r = z.im * 0.15915494309189535f;  // revolutions
+    r -= __builtin_rintf(r);
+    float s = __builtin_amdgcn_sinf(r), c = __builtin_amdgcn_cosf(r);
+    float inv = __builtin_amdgcn_rcpf(c * c + th * th * s * s);
+    return {th * inv, s * c * sech2 * inv};
+}
+
 template <typename R>
 __device__ inline cplx<R> ctanh(cplx<R> z) {
+    if constexpr (sizeof(R) == 4) return ctanh_fast(z);
     R em = M<R>::expm1(-2 * M<R>::fabs(z.re));
     R q = 1 / (2 + em);
     R th = M<R>::copysign(-em * q, z.re);
@@ -165,39 +187,73 @@ __device__ inline uint32_t bounded(uint32_t r, uint32_t n) {
 }
 
 // ------------------------------------------------------ group reductions
-// Butterfly (xor) reductions over aligned groups of L lanes: every lane of
-// the group ends with the bit-identical result, so control flow that
-// depends on it stays uniform within the group.
-template <int L, typename T>
-__device__ inline T group_sum(T v) {
-#pragma unroll
-    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, L);
-    return v;
+// Butterfly reductions over aligned groups of L lanes.  Step O pairs every
+// lane with a partner in the other half of its aligned 2O-lane block:
+// O = 1, 2 by DPP quad permutes (xor 1, xor 2), O = 4, 8 by DPP row
+// half-mirror / mirror (lane i <-> 7-i, 15-i), O = 16 by ds_swizzle xor 16,
+// O = 32 by ds_bpermute.  After steps 1..O every lane of a 2O block holds
+// the same value, so sums and products come out bit-identical in every lane
+// of the group (a + b == b + a) and control flow that depends on them stays
+// uniform within the group.
+template <int O>
+__device__ inline int partner(int v) {
+    if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (O == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    else if constexpr (O == 8) return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    else if constexpr (O == 16) return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+    else return __shfl_xor(v, 32);
 }
-template <int L, typename R>
-__device__ inline cplx<R> group_csum(cplx<R> v) {
-#pragma unroll
-    for (int o = 1; o < L; o <<= 1) {
-        v.re += __shfl_xor(v.re, o, L);
-        v.im += __shfl_xor(v.im, o, L);
+template <int O>
+__device__ inline float partner(float v) {
+    return __int_as_float(partner<O>(__float_as_int(v)));
+}
+template <int O>
+__device__ inline double partner(double v) {
+    int2 h = *reinterpret_cast<int2*>(&v);
+    h.x = partner<O>(h.x);
+    h.y = partner<O>(h.y);
+    return *reinterpret_cast<double*>(&h);
+}
+
+template <int L, int O = 1, typename T>
+__device__ inline T group_sum(T v) {
+    if constexpr (O < L) {
+        v += partner<O>(v);
+        return group_sum<L, 2 * O>(v);
+    } else {
+        return v;
     }
-    return v;
+}
+template <int L, int O = 1, typename R>
+__device__ inline cplx<R> group_csum(cplx<R> v) {
+    if constexpr (O < L) {
+        v.re += partner<O>(v.re);
+        v.im += partner<O>(v.im);
+        return group_csum<L, 2 * O>(v);
+    } else {
+        return v;
+    }
 }
 // product of complex numbers over the group with exponent tracking; the
 // result is returned as mantissa m and exponent e (value m * 2^e).  With
 // FMA contraction the lanes of a group may differ in the last bit of the
 // result, so callers use it as data only (never for control flow).
-template <int L, typename R>
-__device__ inline void group_cprod(cplx<R>& m, int& e) {
-    renorm(m, e);
-#pragma unroll
-    for (int o = 1; o < L; o <<= 1) {
-        cplx<R> q{__shfl_xor(m.re, o, L), __shfl_xor(m.im, o, L)};
-        int qe = __shfl_xor(e, o, L);
+template <int L, int O = 1, typename R>
+__device__ inline void group_cprod_steps(cplx<R>& m, int& e) {
+    if constexpr (O < L) {
+        cplx<R> q{partner<O>(m.re), partner<O>(m.im)};
+        int qe = partner<O>(e);
         m = m * q;
         e += qe;
         renorm(m, e);
+        group_cprod_steps<L, 2 * O>(m, e);
     }
+}
+template <int L, typename R>
+__device__ inline void group_cprod(cplx<R>& m, int& e) {
+    renorm(m, e);
+    group_cprod_steps<L>(m, e);
 }
 
 }  // namespace rbmgpu

@@ -11,6 +11,9 @@ namespace rbmgpu {
 struct Dims {
     int N, H, n_alpha, n_symm, n_vb, nw;
     int w_lds;  // 1: the W table is staged in LDS too (else read from memory)
+    int t_lds;  // 1: the local-operator tables are staged in LDS too
+    int npb;    // proposal bonds
+    int n_terms, n_rowptr, n_ent;  // sizes of the local-operator tables
 };
 
 // weight tables, all complex R; index k*n_alpha + alpha for the N x n_alpha
@@ -68,6 +71,7 @@ struct EvalArgs {
     double* obs;     // optional
     cplx<R>* Tout;   // optional
     double* ratio;   // optional, with flips
+    const int32_t* pb;
     int32_t flips[8];
     int n_flips;
 };
@@ -146,8 +150,14 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <typename R>
 size_t lds_bytes(const Dims& d) {
-    return (d.w_lds ? 3 : 2) * (size_t)d.N * d.n_alpha * sizeof(cplx<R>) +
-           (d.n_symm > 1 ? (size_t)d.n_symm * d.N : 0);
+    // [C | S | W? | rb] complex R, [val?] complex double, [rowptr? k? group?
+    // rowbase? | pb] int32, [sites? | mask? | perm] bytes
+    size_t b = ((d.w_lds ? 3 : 2) * (size_t)d.N * d.n_alpha + d.N) * sizeof(cplx<R>);
+    if (d.t_lds) b += (size_t)d.n_ent * 16 + 4 * ((size_t)d.n_rowptr + 3 * d.n_terms) + 8 * (size_t)d.n_terms +
+                      d.n_ent;
+    b += 8 * (size_t)d.npb;
+    b += d.n_symm > 1 ? (size_t)d.n_symm * d.N : 0;
+    return (b + 15) & ~(size_t)15;
 }
 
 template <typename R>

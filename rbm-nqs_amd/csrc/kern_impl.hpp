@@ -31,25 +31,47 @@ struct Lds {
     const cplx<R>* S;
     const cplx<R>* W;
     const uint8_t* P;
+    const cplx<R>* rb;  // ratio bias per site
+    const int32_t* pb;  // proposal bonds (a, b)
+    Terms T;            // local operators (LDS copies when d.t_lds)
 };
 
-// stage the weight tables and symmetry permutations into LDS
+template <typename T>
+__device__ inline T* lds_copy(char*& at, const T* src, size_t n) {
+    T* dst = reinterpret_cast<T*>(at);
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    at += n * sizeof(T);
+    return dst;
+}
+
+// stage the weight tables, visible biases, proposal bonds, local-operator
+// tables and symmetry permutations into LDS (layout of lds_bytes)
 template <typename R>
-__device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, char* smem) {
-    int nw = d.N * d.n_alpha;
-    cplx<R>* C = reinterpret_cast<cplx<R>*>(smem);
-    cplx<R>* S = C + nw;
-    cplx<R>* W = d.w_lds ? S + nw : const_cast<cplx<R>*>(t.W);
-    uint8_t* P = reinterpret_cast<uint8_t*>(S + (d.w_lds ? 2 : 1) * nw);
-    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
-        C[i] = t.C[i];
-        S[i] = t.S[i];
-        if (d.w_lds) W[i] = t.W[i];
+__device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, const Terms& T, const int32_t* pb,
+                               char* smem) {
+    const size_t nw = (size_t)d.N * d.n_alpha;
+    char* at = smem;
+    Lds<R> l;
+    l.C = lds_copy(at, t.C, nw);
+    l.S = lds_copy(at, t.S, nw);
+    l.W = d.w_lds ? lds_copy(at, t.W, nw) : t.W;
+    l.rb = lds_copy(at, t.rb, d.N);
+    l.T = T;
+    if (d.t_lds) {
+        l.T.val = lds_copy(at, T.val, 2 * (size_t)d.n_ent);
+        l.T.rowptr = lds_copy(at, T.rowptr, d.n_rowptr);
+        l.T.k = lds_copy(at, T.k, d.n_terms);
+        l.T.group = lds_copy(at, T.group, d.n_terms);
+        l.T.rowbase = lds_copy(at, T.rowbase, d.n_terms);
     }
-    if (d.n_symm > 1)
-        for (int i = threadIdx.x; i < d.n_symm * d.N; i += blockDim.x) P[i] = t.perm[i];
+    l.pb = lds_copy(at, pb, 2 * (size_t)d.npb);
+    if (d.t_lds) {
+        l.T.sites = lds_copy(at, T.sites, 8 * (size_t)d.n_terms);
+        l.T.mask = lds_copy(at, T.mask, d.n_ent);
+    }
+    l.P = d.n_symm > 1 ? lds_copy(at, t.perm, (size_t)d.n_symm * d.N) : t.perm;
     __syncthreads();
-    return {C, S, W, P};
+    return l;
 }
 
 __device__ inline int kidx(const Dims& d, const uint8_t* P, int s, int f) {
@@ -151,13 +173,12 @@ struct Units {
 // visible-bias factor exp(-2 sum sigma_f c_f) (rbm_base.cpp:306-318) times
 // the group product of the per-unit cosh ratios
 template <typename R, int L, int K>
-__device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Tabs<R>& t,
-                                          const Units<R, L, K>& U, const int* fl, const R* sg,
-                                          int mask) {
+__device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
+                                          const int* fl, const R* sg, int mask) {
     cplx<R> bias{0, 0};
 #pragma unroll
     for (int i = 0; i < 8; i++)
-        if ((mask >> i) & 1) bias = bias + (-2 * sg[i]) * t.rb[fl[i]];
+        if ((mask >> i) & 1) bias = bias + (-2 * sg[i]) * l.rb[fl[i]];
     cplx<R> p;
     int pe;
     U.ratio_mask(d, l, fl, sg, mask, p, pe);
@@ -170,8 +191,9 @@ __device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const 
 // local observables (local_op.cpp:31-60, summed per group as
 // local_op_chain.cpp:54-60); lane 0 writes out[2*g], out[2*g+1]
 template <typename R, int L, int K>
-__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Tabs<R>& t, const Terms& T,
-                                const Units<R, L, K>& U, const uint64_t* st, int ln, double* out) {
+__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U, const uint64_t* st,
+                                int ln, double* out) {
+    const Terms& T = l.T;
     cplx<double> acc{0, 0};
     int g = T.n_terms > 0 ? T.group[0] : 0;
     for (int ti = 0; ti < T.n_terms; ti++) {
@@ -203,7 +225,7 @@ __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Tabs<R>& t
             if (mask == 0) {
                 acc = acc + v;
             } else {
-                acc = acc + v * term_ratio<R, L, K>(d, l, t, U, fl, sg, mask);
+                acc = acc + v * term_ratio<R, L, K>(d, l, U, fl, sg, mask);
             }
         }
     }
@@ -223,7 +245,7 @@ template <typename R, int L, int K>
 __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     const Dims d = a.d;
-    Lds<R> l = stage<R>(d, a.t, smem);
+    Lds<R> l = stage<R>(d, a.t, a.T, a.pb, smem);
     int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     int chain = gtid / L, ln = gtid % L;
     if (chain >= a.n_chains) return;
@@ -246,20 +268,40 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
     U.init(d, ln);
     U.from_state(d, l, a.t.hb, st);
 
+    // The proposals of slots sw0 .. sw0 + L - 1 of a step are drawn ahead,
+    // one Philox block per lane (lane ln: slot sw0 + ln), and read by the
+    // group with a lane shuffle: code = f0 | (f1 + 1) << 8, f1 = -1 for a
+    // single flip, and the acceptance uniform.
+    int pcode = 0;
+    R pu = 0;
+    auto draw = [&](int64_t step, int sw0) {
+        u4 r = philox((uint32_t)chain, a.epoch, (uint32_t)step, (uint32_t)(sw0 + ln), a.k0, a.k1);
+        int f0, f1 = -1;
+        if ((uint64_t)r.x < a.thr && d.npb > 0) {
+            int bi = (int)bounded(r.y, (uint32_t)d.npb);
+            f0 = l.pb[2 * bi];
+            f1 = l.pb[2 * bi + 1];
+        } else {
+            f0 = (int)bounded(r.y, (uint32_t)d.N);
+        }
+        pcode = f0 | ((f1 + 1) << 8);
+        pu = uniform<R>(r.z, r.w);
+    };
+
     uint32_t acc = 0, prop = 0;
     int64_t k = 0;
     for (int64_t step = 0; step < total; step++) {
+        int sw0 = 0;
+        draw(step, 0);
         for (int sw = 0; sw < d.N; sw++) {
-            u4 r = philox((uint32_t)chain, a.epoch, (uint32_t)step, (uint32_t)sw, a.k0, a.k1);
-            int f0, f1 = -1;
-            if ((uint64_t)r.x < a.thr && a.npb > 0) {
-                int bi = (int)bounded(r.y, (uint32_t)a.npb);
-                f0 = a.pb[2 * bi];
-                f1 = a.pb[2 * bi + 1];
-                sw++;
-            } else {
-                f0 = (int)bounded(r.y, (uint32_t)d.N);
+            if (sw >= sw0 + L) {
+                sw0 = sw & ~(L - 1);
+                draw(step, sw0);
             }
+            int code = __shfl(pcode, sw - sw0, L);
+            R uu = __shfl(pu, sw - sw0, L);
+            int f0 = code & 0xFF, f1 = (code >> 8) - 1;
+            if (f1 >= 0) sw++;
             prop++;
             R s0 = bitof(st, f0) ? (R)-1 : (R)1;
             R s1 = (f1 >= 0 && bitof(st, f1)) ? (R)-1 : (R)1;
@@ -268,10 +310,9 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
             U.ratio12(d, l, f0, s0, f1, s1, p, pe);
             R lp = M<R>::log(norm2(p)) + (R)(2 * pe) * (R)0.69314718055994530942;
             R lnr2 = group_sum<L>(lp);
-            R bre = (-2 * s0) * a.t.rb[f0].re;
-            if (f1 >= 0) bre += (-2 * s1) * a.t.rb[f1].re;
+            R bre = (-2 * s0) * l.rb[f0].re;
+            if (f1 >= 0) bre += (-2 * s1) * l.rb[f1].re;
             lnr2 += 2 * bre;
-            R uu = uniform<R>(r.z, r.w);
             if (uu < M<R>::exp(lnr2)) {
                 U.accept12(d, l, f0, s0, f1, s1);
                 st[f0 >> 6] ^= (uint64_t)1 << (f0 & 63);
@@ -286,7 +327,7 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
 #pragma unroll
             for (int u = 0; u < K; u++)
                 if (U.a[u] >= 0) a.Tout[smp * d.H + ln + u * L] = U.tn[u];
-            eval_obs<R, L, K>(d, l, a.t, a.T, U, st, ln, a.obs + smp * 2 * a.T.n_groups);
+            eval_obs<R, L, K>(d, l, U, st, ln, a.obs + smp * 2 * a.T.n_groups);
             k++;
         }
     }
@@ -303,7 +344,7 @@ template <typename R, int L, int K>
 __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     const Dims d = a.d;
-    Lds<R> l = stage<R>(d, a.t, smem);
+    Lds<R> l = stage<R>(d, a.t, a.T, a.pb, smem);
     int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     int64_t g = gtid / L;
     int ln = gtid % L;
@@ -334,7 +375,7 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
             a.logpsi[2 * g + 1] = s.im;
         }
     }
-    if (a.obs) eval_obs<R, L, K>(d, l, a.t, a.T, U, st, ln, a.obs + g * 2 * a.T.n_groups);
+    if (a.obs) eval_obs<R, L, K>(d, l, U, st, ln, a.obs + g * 2 * a.T.n_groups);
     if (a.ratio) {
         int fl[8];
         R sg[8];
@@ -345,7 +386,7 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
             sg[i] = bitof(st, fl[i]) ? (R)-1 : (R)1;
             if (i < a.n_flips) mask |= 1 << i;
         }
-        cplx<double> r = term_ratio<R, L, K>(d, l, a.t, U, fl, sg, mask);
+        cplx<double> r = term_ratio<R, L, K>(d, l, U, fl, sg, mask);
         if (ln == 0) {
             a.ratio[2 * g] = r.re;
             a.ratio[2 * g + 1] = r.im;

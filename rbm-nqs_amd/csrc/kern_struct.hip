@@ -21,8 +21,11 @@ static bool pick(int N, int M, int& ct, int& rt) {
 }
 
 static size_t lds_for(int ct, int rt) {
-    size_t Mp = 16 * ct, Np = 16 * rt;
-    return 2 * Np * Mp * 4 + (size_t)kTS * Np * 4 + (size_t)kTS * Mp * 8 + kTS * 8 + 2 * Np * 4 + 2 * Mp * 4;
+#define X(C_, R_) \
+    if (ct == C_ && rt == R_) return StructLds<C_, R_>::bytes;
+    STRUCT_SHAPES(X)
+#undef X
+    return 0;
 }
 
 size_t struct_lds_bytes(int N, int M) {

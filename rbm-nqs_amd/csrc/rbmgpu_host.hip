@@ -25,7 +25,7 @@ struct rbmgpu_ctx {
     int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0;
     int64_t P = 0, P_rbm = 0, ld = 0;  // ld: O row length (P rounded up to even)
     std::vector<int32_t> perm, basis, pb;
-    int n_terms = 0, n_groups = 1;
+    int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0;
     // ---- sampler
     int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
     int64_t n_samples = 0;
@@ -116,10 +116,20 @@ int upload(rbmgpu_ctx* ctx, T** p, const void* src, size_t bytes) {
 
 size_t rsize(const rbmgpu_ctx* c) { return c->prec == RBMGPU_FP32 ? 4 : 8; }
 
+// LDS staging: everything if it fits, else drop the local-operator tables,
+// then W (both are then read from memory)
 Dims dims(const rbmgpu_ctx* c) {
-    Dims d{c->N, c->H, c->na, c->nsymm, c->nvb, c->nw, 1};
-    size_t need = (c->prec == RBMGPU_FP32 ? lds_bytes<float>(d) : lds_bytes<double>(d));
-    if (need > c->lds_limit) d.w_lds = 0;
+    Dims d{c->N, c->H, c->na, c->nsymm, c->nvb, c->nw, 1, 1, (int)c->pb.size() / 2, c->n_terms, c->n_rowptr,
+           c->n_ent};
+    auto fits = [&] {
+        return (c->prec == RBMGPU_FP32 ? lds_bytes<float>(d) : lds_bytes<double>(d)) <= c->lds_limit;
+    };
+    const int order[4][2] = {{1, 1}, {1, 0}, {0, 1}, {0, 0}};
+    for (auto& o : order) {
+        d.w_lds = o[0];
+        d.t_lds = o[1];
+        if (fits()) break;
+    }
     return d;
 }
 
@@ -272,16 +282,17 @@ int autotune(rbmgpu_ctx* c) {
         c->block = pick_block(c, c->shape.L, c->n_chains);
         return 0;
     }
-    // time a short epoch (1 sample per chain, 2 warmup sweeps) per shape
-    if (int rc = ensure_samples(c, c->n_chains)) return rc;
+    // time a short epoch (2 samples per chain, 2 warmup sweeps) per shape
+    if (int rc = ensure_samples(c, 2 * (int64_t)c->n_chains)) return rc;
     float best_ms = 1e30f;
     for (Shape s : cand) {
         int block = pick_block(c, s.L, c->n_chains);
         float ms = 0;
         for (int rep = 0; rep < 2; rep++) {
             HIPCK(hipEventRecord(c->ev[10], c->stream));
-            int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, 0xFFFFFFF0u, s, block, c->n_chains, 2)
-                                            : run_sample<double>(c, 0xFFFFFFF0u, s, block, c->n_chains, 2);
+            int64_t ns = 2 * (int64_t)c->n_chains;
+            int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, 0xFFFFFFF0u, s, block, ns, 2)
+                                            : run_sample<double>(c, 0xFFFFFFF0u, s, block, ns, 2);
             if (rc) return rc;
             HIPCK(hipEventRecord(c->ev[11], c->stream));
             HIPCK(hipEventSynchronize(c->ev[11]));
@@ -571,9 +582,9 @@ int ensure_sx(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
     int64_t blocks, row;
     if (c->structured) {
-        // tiles of 64 samples, a few tiles per block
+        // tiles of 64 samples; two resident blocks per CU walk them
         int64_t tiles = (c->S + 63) / 64;
-        blocks = std::min<int64_t>(tiles, 4 * (int64_t)c->n_cu);
+        blocks = std::min<int64_t>(tiles, 2 * (int64_t)c->n_cu);
         row = c->ld_s;
     } else {
         if (c->ld > sx_max_params()) return fail(c, "too many parameters for the S.x kernel");
@@ -745,6 +756,7 @@ int run_eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi,
     a.obs = obs;
     a.Tout = (cplx<R>*)Tout;
     a.ratio = ratio;
+    a.pb = c->d_pb;
     a.n_flips = nf;
     for (int i = 0; i < nf; i++) a.flips[i] = flips[i];
     Shape s = c->shape.L ? c->shape : Shape{0, 0};
@@ -888,6 +900,8 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (upload(ctx, &c->d_tgroup, tgroup.data(), tgroup.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_rowbase, rowbase.data(), rowbase.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_rowptr, rowptr.data(), rowptr.size() * 4)) return bad(c->err);
+    c->n_rowptr = (int)rowptr.size();
+    c->n_ent = (int)mask.size();
     if (upload(ctx, &c->d_tsites, tsites.data(), tsites.size())) return bad(c->err);
     if (upload(ctx, &c->d_mask, mask.data(), mask.size())) return bad(c->err);
     if (upload(ctx, &c->d_val, val.data(), val.size() * 8)) return bad(c->err);
