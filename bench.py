@@ -25,6 +25,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "samples/s and SR epoch ms at fixed energy error, plain vs learned proposal"
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFS = 157.3
 
 WORKLOAD = dict(model="toric", n_cells=4, rbm="basic", alpha=2, chains=4096, samples_per_chain=64,
                 warmup=100, bond_flips=0.5, sr_max_iterations=100, precision=32, weights_std=0.05)
@@ -121,13 +122,62 @@ def cpu_baseline():
                        f"sampling {st['t_sample']:.2f} s, SR {st['t_opt']:.2f} s")
 
 
+# HBM bytes per k_struct launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (KiB per dispatch) of this workload, scaled by the copy-kernel factor
+# (bytes a plain copy moves / what the counters report for it); None until
+# measured for the current kernel
+STRUCT_TRAFFIC_BYTES = None
+
+
+def sx_roofline(g, m, S, sx_ms):
+    """Roofline of one S.x of the CG.
+
+    Structured (basic RBM, FP32): k_struct is two real-by-complex GEMMs per
+    tile, G = Sigma X and Z = Sigma^T diag(y) conj(T), 8 S N M flops, plus
+    the O(S M) y / column-sum work (26 S M + 8 S N); MFMA bound against the
+    dense FP32 MFMA peak.  Materialised O: one pass over O (P x S complex
+    FP32), HBM bound."""
+    M = (m.P - m.N) // (m.N + 1)  # P = N + M + N M for the basic RBM
+    if g.structured():
+        flops = 8 * S * m.N * M + 26 * S * M + 8 * S * m.N
+        achieved = flops / (sx_ms * 1e-3) / 1e12 if sx_ms > 0 else 0.0
+        return {"kernel": "k_struct (one S.x of the CG: G = Sigma X, Z = Sigma^T diag(y) conj(T))",
+                "bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": achieved / FP32_MFMA_PEAK_TFS, "traffic": STRUCT_TRAFFIC_BYTES,
+                "flops_per_launch": flops, "launch_ms": sx_ms}
+    o_bytes = g.P * S * 8
+    achieved = o_bytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
+    return {"kernel": "k_sx_partial (one S.x of the CG over the materialised O)", "bound": "hbm",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "launch_ms": sx_ms}
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box here has one GPU, and the "
               "multi-GPU path is a design (chains shard across GPUs), not a run.", file=sys.stderr)
@@ -136,10 +186,7 @@ def main():
     S = WORKLOAD["chains"] * WORKLOAD["samples_per_chain"]
     ms = 1e3 * dt / a.steps
     value = S * a.steps / dt
-    # dominant kernel: S.x = one pass over the FP32 O matrix (P x S complex)
-    sx_ms = times["sx"]
-    o_bytes = g.P * S * 8
-    achieved = o_bytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
+    roofline = sx_roofline(g, m, S, times["sx"])
     last = log[-1]
     tau = float(np.mean([l["tau"] for l in log[a.warmup:]]))
     out = {
@@ -170,15 +217,7 @@ def main():
         "cg_iterations": last["itn"],
         "kernel_ms": times,
         "launch_shape": list(g.launch_shape()),
-        "roofline": {
-            "kernel": "k_sx_partial + k_sx_finish (one S.x of the CG)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-        },
+        "roofline": roofline,
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

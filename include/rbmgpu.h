@@ -90,8 +90,10 @@ void rbmgpu_destroy(rbmgpu_ctx* ctx);
 const char* rbmgpu_last_error(const rbmgpu_ctx* ctx);
 /* abstract_machine::get_n_params / get_n_neural_params (abstract_machine.hpp:86-101) */
 int rbmgpu_n_params(const rbmgpu_ctx* ctx, int64_t* n_params, int64_t* n_neural);
-/* chosen launch shape: [lanes per chain, units per lane, block] */
-int rbmgpu_launch_shape(const rbmgpu_ctx* ctx, int32_t* out3);
+/* execution layout (no counterpart in the reference): [lanes per chain,
+ * units per lane, block size, S.x path: 1 structured (no O matrix), 0 O
+ * materialised] */
+int rbmgpu_launch_shape(const rbmgpu_ctx* ctx, int32_t* out4);
 
 /* rbm_base::initialize_weights / load (rbm_base.cpp:58-113, 249-288) */
 int rbmgpu_set_params(rbmgpu_ctx* ctx, const double* params);

@@ -949,6 +949,7 @@ int rbmgpu_launch_shape(const rbmgpu_ctx* c, int32_t* o) {
     o[0] = c->shape.L;
     o[1] = c->shape.K;
     o[2] = c->block;
+    o[3] = c->structured ? 1 : 0;
     return 0;
 }
 

@@ -183,7 +183,15 @@ class Gpu:
             raise RuntimeError("rbmgpu: " + libs()[0].rbmgpu_last_error(self.c).decode())
 
     def launch_shape(self):
-        o = np.zeros(3, dtype=np.int32)
+        """(lanes per chain, units per lane, block size)"""
+        return self._layout()[:3]
+
+    def structured(self):
+        """True if S.x runs the structured path (no O matrix)"""
+        return bool(self._layout()[3])
+
+    def _layout(self):
+        o = np.zeros(4, dtype=np.int32)
         self._chk(libs()[0].rbmgpu_launch_shape(self.c, _p(o, I32P)))
         return tuple(int(x) for x in o)
 
