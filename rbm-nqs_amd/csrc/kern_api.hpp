@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "dev_common.hpp"
+#include "kern_pf.hpp"
 
 namespace rbmgpu {
 
@@ -14,6 +15,8 @@ struct Dims {
     int t_lds;  // 1: the local-operator tables are staged in LDS too
     int npb;    // proposal bonds
     int n_terms, n_rowptr, n_ent;  // sizes of the local-operator tables
+    int pf_rows;  // Pfaffian fs_ rows (0: no Pfaffian factor)
+    int waves;    // wavefronts per block (per-wave Pfaffian scratch)
 };
 
 // weight tables, all complex R; index k*n_alpha + alpha for the N x n_alpha
@@ -27,6 +30,8 @@ struct Tabs {
     const cplx<R>* rb;  // ratio bias per site: v_{f mod n_vb} (rbm_base.cpp:310)
     const cplx<R>* lb;  // log-psi bias per site: sum_i v_i #{s: perm_s(basis_i) = f}
     const uint8_t* perm;
+    const cplx<R>* fs;  // Pfaffian weights, pf_rows x 4 (fs[r*4 + c])
+    const int16_t* bs;  // Pfaffian index map, N x N (bs[i*N + j])
 };
 
 // local operators as sparse rows: entries [rowptr[rowbase[t] + loc],
@@ -52,10 +57,12 @@ struct SampArgs {
     int warmup, step, npb;
     uint64_t thr;  // bond flip if philox word < thr (thr = bond_flips * 2^32)
     uint32_t epoch, k0, k1;
+    int pf_refresh;  // fresh Pfaffian inverse every pf_refresh accepted moves (0: never)
     const int32_t* pb;
     uint64_t* states;
     double* obs;
     cplx<R>* Tout;
+    cplx<R>* Opf;    // S x 4 pf_rows: d ln Pf / d fs per kept sample
     uint32_t* acc;
     uint32_t* prop;
 };
@@ -71,6 +78,7 @@ struct EvalArgs {
     double* obs;     // optional
     cplx<R>* Tout;   // optional
     double* ratio;   // optional, with flips
+    cplx<R>* Opf;    // optional (Pfaffian): n x 4 pf_rows
     const int32_t* pb;
     int32_t flips[8];
     int n_flips;
@@ -85,6 +93,7 @@ struct OArgs {
     const cplx<R>* T;
     const int32_t* vb_site;  // n_symm x n_vb: perm_s(basis_i)
     const int32_t* invperm;  // n_symm x N
+    const cplx<R>* Opf;      // S x (P - P_rbm) Pfaffian block, or null
     cplx<R>* O;
 };
 
@@ -150,9 +159,13 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <typename R>
 size_t lds_bytes(const Dims& d) {
-    // [C | S | W? | rb] complex R, [val?] complex double, [rowptr? k? group?
-    // rowbase? | pb] int32, [sites? | mask? | perm] bytes
+    // [C | S | W? | rb | fs? | Pfaffian scratch?] complex R, [val?] complex
+    // double, [rowptr? k? group? rowbase? | pb] int32, [bs?] int16,
+    // [sites? | mask? | perm] bytes
     size_t b = ((d.w_lds ? 3 : 2) * (size_t)d.N * d.n_alpha + d.N) * sizeof(cplx<R>);
+    if (d.pf_rows)
+        b += (4 * (size_t)d.pf_rows + (size_t)d.waves * pf_scratch(d.N, d.pf_rows)) * sizeof(cplx<R>) +
+             2 * (size_t)d.N * d.N;
     if (d.t_lds) b += (size_t)d.n_ent * 16 + 4 * ((size_t)d.n_rowptr + 3 * d.n_terms) + 8 * (size_t)d.n_terms +
                       d.n_ent;
     b += 8 * (size_t)d.npb;
@@ -162,6 +175,8 @@ size_t lds_bytes(const Dims& d) {
 
 template <typename R>
 hipError_t launch_sample(const SampArgs<R>& a, Shape s, int block, hipStream_t st);
+// the Pfaffian factor runs one chain per wavefront
+inline bool shape_ok_pf(Shape s, int N) { return s.L == 64 && N <= kPfMaxN; }
 template <typename R>
 hipError_t launch_eval(const EvalArgs<R>& a, Shape s, int block, hipStream_t st);
 template <typename R>

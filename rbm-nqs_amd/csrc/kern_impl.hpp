@@ -34,6 +34,7 @@ struct Lds {
     const cplx<R>* rb;  // ratio bias per site
     const int32_t* pb;  // proposal bonds (a, b)
     Terms T;            // local operators (LDS copies when d.t_lds)
+    PfLds<R> pf;        // Pfaffian tables and this wave's scratch (d.pf_rows > 0)
 };
 
 template <typename T>
@@ -56,6 +57,19 @@ __device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, const Terms& T, 
     l.S = lds_copy(at, t.S, nw);
     l.W = d.w_lds ? lds_copy(at, t.W, nw) : t.W;
     l.rb = lds_copy(at, t.rb, d.N);
+    l.pf = {};
+    if (d.pf_rows) {
+        l.pf.fs = lds_copy(at, t.fs, 4 * (size_t)d.pf_rows);
+        const int sw = pf_scratch(d.N, d.pf_rows);
+        cplx<R>* scr = reinterpret_cast<cplx<R>*>(at) + (size_t)(threadIdx.x >> 6) * sw;
+        at += (size_t)d.waves * sw * sizeof(cplx<R>);
+        l.pf.N = d.N;
+        l.pf.rows = d.pf_rows;
+        l.pf.X = scr;
+        l.pf.D = scr + pf_xsize(d.N, d.pf_rows);
+        l.pf.G = l.pf.D + 8 * 32;
+        l.pf.F = reinterpret_cast<int*>(l.pf.G + 16 * 16);
+    }
     l.T = T;
     if (d.t_lds) {
         l.T.val = lds_copy(at, T.val, 2 * (size_t)d.n_ent);
@@ -65,6 +79,7 @@ __device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, const Terms& T, 
         l.T.rowbase = lds_copy(at, T.rowbase, d.n_terms);
     }
     l.pb = lds_copy(at, pb, 2 * (size_t)d.npb);
+    if (d.pf_rows) l.pf.bs = lds_copy(at, t.bs, (size_t)d.N * d.N);
     if (d.t_lds) {
         l.T.sites = lds_copy(at, T.sites, 8 * (size_t)d.n_terms);
         l.T.mask = lds_copy(at, T.mask, d.n_ent);
@@ -172,9 +187,10 @@ struct Units {
 // psi(sigma')/psi(sigma) for the sites of a term selected by mask: the
 // visible-bias factor exp(-2 sum sigma_f c_f) (rbm_base.cpp:306-318) times
 // the group product of the per-unit cosh ratios
-template <typename R, int L, int K>
+template <typename R, int L, int K, bool PF>
 __device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
-                                          const int* fl, const R* sg, int mask) {
+                                          const cplx<R> (&pv)[16], const uint64_t* st, int ln, const int* fl,
+                                          const R* sg, int mask) {
     cplx<R> bias{0, 0};
 #pragma unroll
     for (int i = 0; i < 8; i++)
@@ -185,14 +201,20 @@ __device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const 
     group_cprod<L>(p, pe);
     cplx<R> r = cexp(bias) * p;
     double sc = ::ldexp(1.0, pe);
-    return {(double)r.re * sc, (double)r.im * sc};
+    cplx<double> out{(double)r.re * sc, (double)r.im * sc};
+    if constexpr (PF) {  // times Pf(A(sigma')) / Pf(A(sigma)) (pfaffian.hpp:73-86)
+        PfProp<R, false> pp;
+        cplx<R> q = pf_ratio<R, false>(l.pf, pv, st, fl, mask, ln, pp);
+        out = out * cplx<double>{(double)q.re, (double)q.im};
+    }
+    return out;
 }
 
 // local observables (local_op.cpp:31-60, summed per group as
 // local_op_chain.cpp:54-60); lane 0 writes out[2*g], out[2*g+1]
-template <typename R, int L, int K>
-__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U, const uint64_t* st,
-                                int ln, double* out) {
+template <typename R, int L, int K, bool PF>
+__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
+                                const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
     const Terms& T = l.T;
     cplx<double> acc{0, 0};
     int g = T.n_terms > 0 ? T.group[0] : 0;
@@ -225,7 +247,7 @@ __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L
             if (mask == 0) {
                 acc = acc + v;
             } else {
-                acc = acc + v * term_ratio<R, L, K>(d, l, U, fl, sg, mask);
+                acc = acc + v * term_ratio<R, L, K, PF>(d, l, U, pv, st, ln, fl, sg, mask);
             }
         }
     }
@@ -241,7 +263,7 @@ __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L
 // bond_flips both ends of a random non-z bond are flipped (taking two
 // slots), else one uniform site; accept if u < |psi'/psi|^2.  At kept
 // steps the local observables and tanh(theta) of the sample are written.
-template <typename R, int L, int K>
+template <typename R, int L, int K, bool PF>
 __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     const Dims d = a.d;
@@ -267,6 +289,10 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
     Units<R, L, K> U;
     U.init(d, ln);
     U.from_state(d, l, a.t.hb, st);
+    // Pfaffian: inverse of A(sigma) in registers (one chain per wavefront)
+    cplx<R> pv[16];
+    PfProp<R, true> pp;
+    if constexpr (PF) pf_fresh(l.pf, st, ln, pv);
 
     // The proposals of slots sw0 .. sw0 + L - 1 of a step are drawn ahead,
     // one Philox block per lane (lane ln: slot sw0 + ln), and read by the
@@ -313,11 +339,21 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
             R bre = (-2 * s0) * l.rb[f0].re;
             if (f1 >= 0) bre += (-2 * s1) * l.rb[f1].re;
             lnr2 += 2 * bre;
+            if constexpr (PF) {
+                int fl[2] = {f0, f1};
+                cplx<R> q = pf_ratio<R, true>(l.pf, pv, st, fl, f1 >= 0 ? 3 : 1, ln, pp);
+                lnr2 += M<R>::log(norm2(q));
+            }
             if (uu < M<R>::exp(lnr2)) {
                 U.accept12(d, l, f0, s0, f1, s1);
+                if constexpr (PF) pf_accept(l.pf, pv, pp, ln);
                 st[f0 >> 6] ^= (uint64_t)1 << (f0 & 63);
                 if (f1 >= 0) st[f1 >> 6] ^= (uint64_t)1 << (f1 & 63);
                 acc++;
+                // fresh inverse every pf_refresh accepted moves (metropolis_sampler.cpp:240,
+                // abstract_sampler.cpp:103-113)
+                if constexpr (PF)
+                    if (a.pf_refresh > 0 && acc % (uint32_t)a.pf_refresh == 0) pf_fresh(l.pf, st, ln, pv);
             }
         }
         if (step >= a.warmup && (step - a.warmup) % a.step == 0) {
@@ -327,7 +363,8 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
 #pragma unroll
             for (int u = 0; u < K; u++)
                 if (U.a[u] >= 0) a.Tout[smp * d.H + ln + u * L] = U.tn[u];
-            eval_obs<R, L, K>(d, l, U, st, ln, a.obs + smp * 2 * a.T.n_groups);
+            eval_obs<R, L, K, PF>(d, l, U, pv, st, ln, a.obs + smp * 2 * a.T.n_groups);
+            if constexpr (PF) pf_derivative(l.pf, pv, st, ln, a.Opf + smp * 4 * d.pf_rows);
             k++;
         }
     }
@@ -340,7 +377,7 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
 // ---------------------------------------------------------- eval states --
 // one group per host-given state: theta from scratch, tanh, optionally
 // log psi, local observables and the psi ratio for given flips
-template <typename R, int L, int K>
+template <typename R, int L, int K, bool PF>
 __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     const Dims d = a.d;
@@ -354,6 +391,8 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
     Units<R, L, K> U;
     U.init(d, ln);
     U.from_state(d, l, a.t.hb, st);
+    cplx<R> pv[16];
+    if constexpr (PF) pf_fresh(l.pf, st, ln, pv);
     if (a.Tout) {
 #pragma unroll
         for (int u = 0; u < K; u++)
@@ -370,12 +409,16 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
             s = s + sg * a.t.lb[f];
         }
         s = group_csum<L>(s);
+        cplx<double> lp{(double)s.re, (double)s.im};
+        if constexpr (PF) lp = lp + pf_logpf(l.pf, st, ln);  // + ln Pf(A(sigma))
         if (ln == 0) {
-            a.logpsi[2 * g] = s.re;
-            a.logpsi[2 * g + 1] = s.im;
+            a.logpsi[2 * g] = lp.re;
+            a.logpsi[2 * g + 1] = lp.im;
         }
     }
-    if (a.obs) eval_obs<R, L, K>(d, l, U, st, ln, a.obs + g * 2 * a.T.n_groups);
+    if constexpr (PF)
+        if (a.Opf) pf_derivative(l.pf, pv, st, ln, a.Opf + g * 4 * d.pf_rows);
+    if (a.obs) eval_obs<R, L, K, PF>(d, l, U, pv, st, ln, a.obs + g * 2 * a.T.n_groups);
     if (a.ratio) {
         int fl[8];
         R sg[8];
@@ -386,7 +429,7 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
             sg[i] = bitof(st, fl[i]) ? (R)-1 : (R)1;
             if (i < a.n_flips) mask |= 1 << i;
         }
-        cplx<double> r = term_ratio<R, L, K>(d, l, U, fl, sg, mask);
+        cplx<double> r = term_ratio<R, L, K, PF>(d, l, U, pv, st, ln, fl, sg, mask);
         if (ln == 0) {
             a.ratio[2 * g] = r.re;
             a.ratio[2 * g + 1] = r.im;
@@ -413,6 +456,8 @@ __global__ void __launch_bounds__(256) k_build_O(OArgs<R> a) {
     cplx<R>* o = a.O + k * a.ld;
     int nvb = d.n_vb, na = d.n_alpha;
     for (int p = a.P + threadIdx.x; p < a.ld; p += blockDim.x) o[p] = {0, 0};
+    if (a.Opf)  // Pfaffian block (pfaffian.cpp:399-415), written by the sampler
+        for (int p = a.P_rbm + threadIdx.x; p < a.P; p += blockDim.x) o[p] = a.Opf[k * (a.P - a.P_rbm) + p - a.P_rbm];
     for (int p = threadIdx.x; p < a.P_rbm; p += blockDim.x) {
         cplx<R> v{0, 0};
         if (p < nvb) {

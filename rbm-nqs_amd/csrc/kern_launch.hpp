@@ -5,49 +5,67 @@
 
 namespace rbmgpu {
 
-template <typename R, int L, int K>
+template <typename R, int L, int K, bool PF>
 static hipError_t sample_one(const SampArgs<R>& a, int block, hipStream_t st) {
     int64_t threads = (int64_t)a.n_chains * L;
     int grid = (int)((threads + block - 1) / block);
     size_t lds = lds_bytes<R>(a.d);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sample<R, L, K>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_sample<R, L, K, PF>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sample<R, L, K>), dim3(grid), dim3(block), lds, st, a);
+    hipLaunchKernelGGL((k_sample<R, L, K, PF>), dim3(grid), dim3(block), lds, st, a);
     return hipGetLastError();
 }
-template <typename R, int L, int K>
+template <typename R, int L, int K, bool PF>
 static hipError_t eval_one(const EvalArgs<R>& a, int block, hipStream_t st) {
     int64_t threads = a.n * L;
     int grid = (int)((threads + block - 1) / block);
     size_t lds = lds_bytes<R>(a.d);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_eval<R, L, K>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_eval<R, L, K, PF>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_eval<R, L, K>), dim3(grid), dim3(block), lds, st, a);
+    hipLaunchKernelGGL((k_eval<R, L, K, PF>), dim3(grid), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
 #define RBM_SHAPES(X) \
     X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(8, 16) X(16, 2) X(16, 4) X(16, 8) X(16, 16) X(32, 2) \
     X(32, 4) X(32, 8) X(32, 16) X(64, 1) X(64, 2) X(64, 4) X(64, 8)
+// shapes with a Pfaffian instance: one chain per wavefront
+#define PF_SHAPES(X) X(64, 1) X(64, 2) X(64, 4) X(64, 8)
 
 template <typename R>
 hipError_t launch_sample(const SampArgs<R>& a, Shape s, int block, hipStream_t st) {
+    if (a.d.pf_rows) {
+        if (block != 64 * a.d.waves) return hipErrorInvalidValue;
 #define X(L_, K_) \
-    if (s.L == L_ && s.K == K_) return sample_one<R, L_, K_>(a, block, st);
+    if (s.L == L_ && s.K == K_) return sample_one<R, L_, K_, true>(a, block, st);
+        PF_SHAPES(X)
+#undef X
+        return hipErrorInvalidValue;
+    }
+#define X(L_, K_) \
+    if (s.L == L_ && s.K == K_) return sample_one<R, L_, K_, false>(a, block, st);
     RBM_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
 }
 template <typename R>
 hipError_t launch_eval(const EvalArgs<R>& a, Shape s, int block, hipStream_t st) {
+    if (a.d.pf_rows) {
+        if (block != 64 * a.d.waves) return hipErrorInvalidValue;
 #define X(L_, K_) \
-    if (s.L == L_ && s.K == K_) return eval_one<R, L_, K_>(a, block, st);
+    if (s.L == L_ && s.K == K_) return eval_one<R, L_, K_, true>(a, block, st);
+        PF_SHAPES(X)
+#undef X
+        return hipErrorInvalidValue;
+    }
+#define X(L_, K_) \
+    if (s.L == L_ && s.K == K_) return eval_one<R, L_, K_, false>(a, block, st);
     RBM_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

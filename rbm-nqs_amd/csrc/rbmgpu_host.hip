@@ -22,7 +22,7 @@ int rbmgpu::sx_max_params() { return 128 * 24; }
 
 struct rbmgpu_ctx {
     // ---- model
-    int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0;
+    int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0, pf_refresh = 0;
     int64_t P = 0, P_rbm = 0, ld = 0;  // ld: O row length (P rounded up to even)
     std::vector<int32_t> perm, basis, pb;
     int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0;
@@ -39,6 +39,8 @@ struct rbmgpu_ctx {
     void *d_W = nullptr, *d_C = nullptr, *d_S = nullptr, *d_hb = nullptr, *d_rb = nullptr,
          *d_lb = nullptr;
     uint8_t* d_perm = nullptr;
+    int16_t* d_bs = nullptr;  // Pfaffian index map, bs[i*N + j]
+    void* d_fs = nullptr;     // Pfaffian weights in the weight block
     int32_t *d_invperm = nullptr, *d_vbsite = nullptr, *d_pb = nullptr;
     int32_t *d_tk = nullptr, *d_tgroup = nullptr, *d_rowbase = nullptr, *d_rowptr = nullptr;
     uint8_t *d_tsites = nullptr, *d_mask = nullptr;
@@ -49,6 +51,7 @@ struct rbmgpu_ctx {
     double* d_obs = nullptr;
     void* d_T = nullptr;
     void* d_O = nullptr;
+    void* d_Opf = nullptr;  // S x 4 pf_rows Pfaffian derivatives
     uint32_t *d_acc = nullptr, *d_prop = nullptr;
     bool have_chains = false;
     // structured: basic RBM in FP32 without an O matrix (kern_struct.hpp)
@@ -118,9 +121,9 @@ size_t rsize(const rbmgpu_ctx* c) { return c->prec == RBMGPU_FP32 ? 4 : 8; }
 
 // LDS staging: everything if it fits, else drop the local-operator tables,
 // then W (both are then read from memory)
-Dims dims(const rbmgpu_ctx* c) {
+Dims dims(const rbmgpu_ctx* c, int block = 256) {
     Dims d{c->N, c->H, c->na, c->nsymm, c->nvb, c->nw, 1, 1, (int)c->pb.size() / 2, c->n_terms, c->n_rowptr,
-           c->n_ent};
+           c->n_ent, c->pf_rows, block / 64};
     auto fits = [&] {
         return (c->prec == RBMGPU_FP32 ? lds_bytes<float>(d) : lds_bytes<double>(d)) <= c->lds_limit;
     };
@@ -152,7 +155,8 @@ int build_weight_tables(rbmgpu_ctx* c) {
     // one staging block [W | C | S | hb | rb | lb] copied asynchronously on
     // the context stream; the staging buffer lives in the context so the
     // copy may complete after this call returns
-    const size_t nwt = (size_t)N * na, total = 3 * nwt + na + 2 * (size_t)N;
+    const size_t nwt = (size_t)N * na, nfs = 4 * (size_t)c->pf_rows,
+                 total = 3 * nwt + na + 2 * (size_t)N + nfs;
     const size_t rs = rsize(c) * 2;
     if (!c->d_W) {
         char* base = nullptr;
@@ -163,6 +167,7 @@ int build_weight_tables(rbmgpu_ctx* c) {
         c->d_hb = base + 3 * nwt * rs;
         c->d_rb = base + (3 * nwt + na) * rs;
         c->d_lb = base + (3 * nwt + na + N) * rs;
+        c->d_fs = base + (3 * nwt + na + 2 * N) * rs;
     }
     HIPCK(hipStreamSynchronize(c->stream));  // previous copy out of the staging block
     c->staging.resize(total * rs);
@@ -181,6 +186,12 @@ int build_weight_tables(rbmgpu_ctx* c) {
     pack(3 * nwt, hb);
     pack(3 * nwt + na, rb);
     pack(3 * nwt + na + N, lb);
+    if (nfs) {  // fs_(r, c) at parameter P_rbm + r + c * rows -> fs[r*4 + c]
+        std::vector<zd> fs(nfs);
+        for (int r = 0; r < c->pf_rows; r++)
+            for (int q = 0; q < 4; q++) fs[r * 4 + q] = c->params[c->P_rbm + r + (int64_t)q * c->pf_rows];
+        pack(3 * nwt + na + 2 * N, fs);
+    }
     HIPCK(hipMemcpyAsync(c->d_W, c->staging.data(), total * rs, hipMemcpyHostToDevice, c->stream));
     c->moments_valid = false;
     return 0;
@@ -190,7 +201,7 @@ template <typename R>
 Tabs<R> tabs(const rbmgpu_ctx* c) {
     return {(const cplx<R>*)c->d_W,  (const cplx<R>*)c->d_C,  (const cplx<R>*)c->d_S,
             (const cplx<R>*)c->d_hb, (const cplx<R>*)c->d_rb, (const cplx<R>*)c->d_lb,
-            c->d_perm};
+            c->d_perm,               (const cplx<R>*)c->d_fs, c->d_bs};
 }
 Terms terms(const rbmgpu_ctx* c) {
     return {c->n_terms, c->n_groups, c->d_tk,     c->d_tsites, c->d_tgroup,
@@ -205,24 +216,34 @@ int ensure_samples(rbmgpu_ctx* c, int64_t S) {
     if (int rc = dalloc(ctx, &c->d_T, S * c->H * 2 * rsize(c))) return rc;
     if (!c->structured)
         if (int rc = dalloc(ctx, &c->d_O, S * c->ld * 2 * rsize(c))) return rc;
+    if (c->pf_rows)
+        if (int rc = dalloc(ctx, &c->d_Opf, S * 4 * c->pf_rows * 2 * rsize(c))) return rc;
     c->cap = S;
     return 0;
 }
 
-bool shape_fits(const rbmgpu_ctx* c, Shape s) { return s.L * s.K >= c->H; }
+bool shape_fits(const rbmgpu_ctx* c, Shape s) {
+    return s.L * s.K >= c->H && (c->pf_rows == 0 || shape_ok_pf(s, c->N));
+}
 
-// block size: 256 threads unless that leaves fewer than two blocks per CU
+// block size: 256 threads unless that leaves fewer than two blocks per CU;
+// with the Pfaffian also small enough for its per-wave LDS scratch
 int pick_block(const rbmgpu_ctx* c, int L, int64_t groups) {
     int b = 256;
     while (b > 64 && groups * L / b < 2 * c->n_cu) b /= 2;
-    return std::max(b, L);
+    b = std::max(b, L);
+    if (c->pf_rows)
+        while (b > 64 && (c->prec == RBMGPU_FP32 ? lds_bytes<float>(dims(c, b)) : lds_bytes<double>(dims(c, b))) >
+                             c->lds_limit)
+            b /= 2;
+    return b;
 }
 
 template <typename R>
 int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
     rbmgpu_ctx* ctx = c;
     SampArgs<R> a{};
-    a.d = dims(c);
+    a.d = dims(c, block);
     a.t = tabs<R>(c);
     a.T = terms(c);
     a.n_chains = c->n_chains;
@@ -239,6 +260,8 @@ int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samp
     a.states = c->d_states;
     a.obs = c->d_obs;
     a.Tout = (cplx<R>*)c->d_T;
+    a.Opf = (cplx<R>*)c->d_Opf;
+    a.pf_refresh = c->pf_refresh;
     a.acc = c->d_acc;
     a.prop = c->d_prop;
     HIPCK(launch_sample<R>(a, s, block, c->stream));
@@ -259,6 +282,7 @@ int run_build_O(rbmgpu_ctx* c) {
     a.T = (const cplx<R>*)c->d_T;
     a.vb_site = c->d_vbsite;
     a.invperm = c->d_invperm;
+    a.Opf = c->pf_rows ? (const cplx<R>*)c->d_Opf : nullptr;
     a.O = (cplx<R>*)c->d_O;
     HIPCK(launch_build_O<R>(a, c->stream));
     return 0;
@@ -747,7 +771,6 @@ int run_eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi,
              double* ratio, const int32_t* flips, int nf) {
     rbmgpu_ctx* ctx = c;
     EvalArgs<R> a{};
-    a.d = dims(c);
     a.t = tabs<R>(c);
     a.T = terms(c);
     a.n = n;
@@ -756,6 +779,9 @@ int run_eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi,
     a.obs = obs;
     a.Tout = (cplx<R>*)Tout;
     a.ratio = ratio;
+    // the sample-set path (T written into the context) also writes the
+    // Pfaffian derivatives next to it
+    a.Opf = (c->pf_rows && Tout == c->d_T) ? (cplx<R>*)c->d_Opf : nullptr;
     a.pb = c->d_pb;
     a.n_flips = nf;
     for (int i = 0; i < nf; i++) a.flips[i] = flips[i];
@@ -769,6 +795,7 @@ int run_eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi,
     }
     if (!s.L) return fail(c, "no launch shape for this number of hidden units");
     int block = pick_block(c, s.L, n);
+    a.d = dims(c, block);
     HIPCK(launch_eval<R>(a, s, block, c->stream));
     return 0;
 }
@@ -806,8 +833,21 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     c->nw = (c->N + 63) / 64;
     c->P_rbm = (int64_t)c->nvb + c->na + (int64_t)c->na * c->N;
     c->pf_rows = cfg->pf_rows;
-    if (c->pf_rows != 0) return bad("Pfaffian factor: not supported by this build of the library");
-    c->P = c->P_rbm;
+    c->pf_refresh = cfg->pf_refresh;
+    std::vector<int16_t> bs;
+    if (c->pf_rows < 0) return bad("pf_rows must be >= 0");
+    if (c->pf_rows) {
+        if (c->N > kPfMaxN) return bad("the Pfaffian factor supports at most 32 spins");
+        if (!cfg->pf_bs) return bad("pf_bs is required with a Pfaffian factor");
+        bs.resize((size_t)c->N * c->N);
+        for (int i = 0; i < c->N; i++)
+            for (int j = 0; j < c->N; j++) {
+                int v = cfg->pf_bs[i + (int64_t)j * c->N];  // column-major b(i, j)
+                if ((i == j) != (v == 0) || std::abs(v) > c->pf_rows) return bad("invalid Pfaffian index map");
+                bs[(size_t)i * c->N + j] = (int16_t)v;
+            }
+    }
+    c->P = c->P_rbm + 4 * (int64_t)c->pf_rows;
     c->ld = c->P + (c->P & 1);
     if (!cfg->perm || !cfg->basis) return bad("perm and basis tables are required");
     c->perm.assign(cfg->perm, cfg->perm + (int64_t)c->nsymm * c->N);
@@ -896,6 +936,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (upload(ctx, &c->d_invperm, inv.data(), inv.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_vbsite, vbs.data(), vbs.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_pb, c->pb.data(), c->pb.size() * 4)) return bad(c->err);
+    if (c->pf_rows && upload(ctx, &c->d_bs, bs.data(), bs.size() * 2)) return bad(c->err);
     if (upload(ctx, &c->d_tk, tk.data(), tk.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_tgroup, tgroup.data(), tgroup.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_rowbase, rowbase.data(), rowbase.size() * 4)) return bad(c->err);
