@@ -164,7 +164,7 @@ size_t lds_bytes(const Dims& d) {
     // [sites? | mask? | perm] bytes
     size_t b = ((d.w_lds ? 3 : 2) * (size_t)d.N * d.n_alpha + d.N) * sizeof(cplx<R>);
     if (d.pf_rows)
-        b += (4 * (size_t)d.pf_rows + (size_t)d.waves * pf_scratch(d.N, d.pf_rows)) * sizeof(cplx<R>) +
+        b += (4 * (size_t)d.pf_rows + (size_t)d.waves * pf_scratch<R>(d.N, d.pf_rows)) * sizeof(cplx<R>) +
              2 * (size_t)d.N * d.N;
     if (d.t_lds) b += (size_t)d.n_ent * 16 + 4 * ((size_t)d.n_rowptr + 3 * d.n_terms) + 8 * (size_t)d.n_terms +
                       d.n_ent;

@@ -60,13 +60,14 @@ __device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, const Terms& T, 
     l.pf = {};
     if (d.pf_rows) {
         l.pf.fs = lds_copy(at, t.fs, 4 * (size_t)d.pf_rows);
-        const int sw = pf_scratch(d.N, d.pf_rows);
+        const int sw = pf_scratch<R>(d.N, d.pf_rows);
         cplx<R>* scr = reinterpret_cast<cplx<R>*>(at) + (size_t)(threadIdx.x >> 6) * sw;
         at += (size_t)d.waves * sw * sizeof(cplx<R>);
         l.pf.N = d.N;
         l.pf.rows = d.pf_rows;
         l.pf.X = scr;
-        l.pf.D = scr + pf_xsize(d.N, d.pf_rows);
+        l.pf.Xd = reinterpret_cast<cplx<double>*>(scr);
+        l.pf.D = scr + pf_xsize<R>(d.N, d.pf_rows);
         l.pf.G = l.pf.D + 8 * 32;
         l.pf.F = reinterpret_cast<int*>(l.pf.G + 16 * 16);
     }

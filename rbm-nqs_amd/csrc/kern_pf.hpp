@@ -5,9 +5,10 @@
 // matrix A(sigma), columns 16h .. 16h + 15 with h = l >> 5: v[q] =
 // inv(i, 16h + q).  Rows i >= N are zero.
 //
-// Per-wave LDS scratch (complex R): X (N x N, column-major X[j*N + i]) for
-// fresh inverses / Pfaffians and as exchange space, D (8 x 32) for the
-// Delta-a rows of a flip set, G (16 x 16) for the small 2m x 2m matrix.
+// Per-wave LDS scratch: X (N x N complex double, column-major X[j*N + i])
+// for fresh inverses / Pfaffians, which are always formed in FP64, and as
+// exchange space, D (8 x 32) for the Delta-a rows of a flip set, G (16 x 16)
+// for the small 2m x 2m matrix.
 //
 // Ratio and update follow pfaffian.cpp:299-397 (Sherman-Morrison-Woodbury):
 // for flips f_0..f_{m-1}, B = [Delta-a columns | unit columns e_{f_k}],
@@ -29,20 +30,27 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// scratch complex entries per wave: X (max(N^2, 256, 4 rows)), D, G, and
-// 16 ints for the flip list (8 complex slots)
+// scratch per wave in complex R entries: X (room for N x N complex double,
+// 256 entries, or 4 rows accumulators), D (8 x 32), G (16 x 16), and 16
+// ints for the flip list (8 complex slots)
+template <typename R>
 __host__ __device__ inline int pf_xsize(int N, int rows) {
-    int x = N * N > 256 ? N * N : 256;
+    int x = N * N * (int)(sizeof(cplx<double>) / sizeof(cplx<R>));
+    if (x < 256) x = 256;
     return 4 * rows > x ? 4 * rows : x;
 }
-__host__ __device__ inline int pf_scratch(int N, int rows) { return pf_xsize(N, rows) + 8 * 32 + 16 * 16 + 8; }
+template <typename R>
+__host__ __device__ inline int pf_scratch(int N, int rows) {
+    return pf_xsize<R>(N, rows) + 8 * 32 + 16 * 16 + 8;
+}
 
 template <typename R>
 struct PfLds {
     const int16_t* bs;   // N x N, bs[i*N + j] = index-map entry b(i, j)
     const cplx<R>* fs;   // rows x 4: fs[r*4 + c]
     int N, rows;
-    cplx<R>* X;          // this wave's N x N scratch
+    cplx<R>* X;          // this wave's exchange / accumulator scratch
+    cplx<double>* Xd;    // the same region as N x N complex double
     cplx<R>* D;          // 8 x 32
     cplx<R>* G;          // 16 x 16
     int* F;              // 16 ints: the flip list of the current ratio
@@ -86,74 +94,61 @@ __device__ inline int wave_argmax(double key, int lane) {
     return __builtin_amdgcn_readfirstlane(best);
 }
 
-// X = A(sigma) (column-major), all N x N entries
+// Xd = A(sigma) (column-major), all N x N entries, in FP64
 template <typename R>
 __device__ inline void pf_build(const PfLds<R>& l, const uint64_t* st, int lane) {
     const int N = l.N;
     for (int e = lane; e < N * N; e += 64) {
         int i = e % N, j = e / N;
-        l.X[e] = pf_a(l, i, j, bit2(st, i), bit2(st, j));
+        cplx<R> a = pf_a(l, i, j, bit2(st, i), bit2(st, j));
+        l.Xd[e] = {(double)a.re, (double)a.im};
     }
     wave_sync();
 }
 
-// Gauss-Jordan inverse of X in place with partial pivoting (row swaps,
+// Gauss-Jordan inverse of Xd in place (FP64) with partial pivoting (row swaps,
 // columns unscrambled at the end); returns false if singular
 template <typename R>
 __device__ inline bool pf_gj(const PfLds<R>& l, int lane, int& ipiv_lane) {
     const int N = l.N;
-    cplx<R>* X = l.X;
+    cplx<double>* X = l.Xd;
     bool ok = true;
     for (int c = 0; c < N; c++) {
         double key = -1;
         if (lane < N && lane >= c) {
-            cplx<R> x = X[c * N + lane];
+            cplx<double> x = X[c * N + lane];
             key = (double)x.re * x.re + (double)x.im * x.im;
         }
         int p = wave_argmax(key, lane);
         if (lane == c) ipiv_lane = p;
         if (p != c) {
             if (lane < N) {
-                cplx<R> a = X[lane * N + p], b = X[lane * N + c];
+                cplx<double> a = X[lane * N + p], b = X[lane * N + c];
                 X[lane * N + p] = b;
                 X[lane * N + c] = a;
             }
             wave_sync();
         }
-        cplx<R> piv = X[c * N + c];
+        cplx<double> piv = X[c * N + c];
         if (piv.re == 0 && piv.im == 0) ok = false;
-        cplx<R> inv = cdiv(cplx<R>{1, 0}, piv);
+        cplx<double> inv = cdiv(cplx<double>{1, 0}, piv);
         // row c: X(c, c) := 1, then row c *= 1 / piv
         if (lane < N) {
-            cplx<R> t = lane == c ? cplx<R>{1, 0} : X[lane * N + c];
+            cplx<double> t = lane == c ? cplx<double>{1, 0} : X[lane * N + c];
             X[lane * N + c] = t * inv;
         }
         wave_sync();
         // other rows: X(r, j) -= X(r, c) X(c, j), with X(r, c) := 0 first;
-        // every lane reads its operands before any lane writes
-        constexpr int EPL = kPfMaxN * kPfMaxN / 64;  // entries per lane
-        cplx<R> nv[EPL];
-#pragma unroll
-        for (int t = 0; t < EPL; t++) {
-            int e = lane + 64 * t;
-            nv[t] = {0, 0};
-            if (e < N * N) {
-                int r = e % N, j = e / N;
-                cplx<R> x = X[e];
-                if (r != c) {
-                    cplx<R> f = X[c * N + r];
-                    cplx<R> rc = X[j * N + c];
-                    nv[t] = (j == c ? cplx<R>{0, 0} : x) - f * rc;
-                } else {
-                    nv[t] = x;
-                }
-            }
-        }
+        // column c is snapshotted into the G area so no lane reads an
+        // entry another lane has already rewritten (row c is not written)
+        cplx<double>* col = reinterpret_cast<cplx<double>*>(l.G);
+        if (lane < N) col[lane] = X[c * N + lane];
         wave_sync();
-#pragma unroll
-        for (int t = 0; t < EPL; t++) {
-            int e = lane + 64 * t;
-            if (e < N * N) X[e] = nv[t];
+        for (int e = lane; e < N * N; e += 64) {
+            int r = e % N, j = e / N;
+            if (r == c) continue;
+            cplx<double> x = j == c ? cplx<double>{0, 0} : X[e];
+            X[e] = x - col[r] * X[j * N + c];
         }
         wave_sync();
     }
@@ -162,7 +157,7 @@ __device__ inline bool pf_gj(const PfLds<R>& l, int lane, int& ipiv_lane) {
         int p = __shfl(ipiv_lane, c);
         if (p != c) {
             if (lane < N) {
-                cplx<R> a = X[p * N + lane], b = X[c * N + lane];
+                cplx<double> a = X[p * N + lane], b = X[c * N + lane];
                 X[p * N + lane] = b;
                 X[c * N + lane] = a;
             }
@@ -172,7 +167,7 @@ __device__ inline bool pf_gj(const PfLds<R>& l, int lane, int& ipiv_lane) {
     return ok;
 }
 
-// fresh inverse of A(sigma), antisymmetrised (pfaffian.cpp:284-297), into v
+// fresh FP64 inverse of A(sigma), antisymmetrised (pfaffian.cpp:284-297), into v
 template <typename R>
 __device__ inline bool pf_fresh(const PfLds<R>& l, const uint64_t* st, int lane, cplx<R> (&v)[16]) {
     pf_build(l, st, lane);
@@ -184,8 +179,8 @@ __device__ inline bool pf_fresh(const PfLds<R>& l, const uint64_t* st, int lane,
         int j = 16 * h + q;
         cplx<R> r{0, 0};
         if (i < N && j < N) {
-            cplx<R> a = l.X[j * N + i], b = l.X[i * N + j];
-            r = {(R)0.5 * (a.re - b.re), (R)0.5 * (a.im - b.im)};
+            cplx<double> a = l.Xd[j * N + i], b = l.Xd[i * N + j];
+            r = {(R)(0.5 * (a.re - b.re)), (R)(0.5 * (a.im - b.im))};
         }
         v[q] = r;
     }
@@ -199,39 +194,39 @@ template <typename R>
 __device__ inline cplx<double> pf_logpf(const PfLds<R>& l, const uint64_t* st, int lane) {
     pf_build(l, st, lane);
     const int N = l.N;
-    cplx<R>* X = l.X;
+    cplx<double>* X = l.Xd;
     if (N & 1) return {-INFINITY, 0};
     double lre = 0, lim = 0;
     for (int k = 0; k + 1 < N; k += 2) {
         double key = -1;
         if (lane > k && lane < N) {
-            cplx<R> x = X[lane * N + k];  // X(k, lane)
+            cplx<double> x = X[lane * N + k];  // X(k, lane)
             key = (double)x.re * x.re + (double)x.im * x.im;
         }
         int kp = wave_argmax(key, lane);
         if (kp != k + 1) {
             if (lane < N) {  // rows k+1 <-> kp
-                cplx<R> a = X[lane * N + k + 1], b = X[lane * N + kp];
+                cplx<double> a = X[lane * N + k + 1], b = X[lane * N + kp];
                 X[lane * N + k + 1] = b;
                 X[lane * N + kp] = a;
             }
             wave_sync();
             if (lane < N) {  // columns k+1 <-> kp
-                cplx<R> a = X[(k + 1) * N + lane], b = X[kp * N + lane];
+                cplx<double> a = X[(k + 1) * N + lane], b = X[kp * N + lane];
                 X[(k + 1) * N + lane] = b;
                 X[kp * N + lane] = a;
             }
             wave_sync();
             lim += M_PI;  // sign flip
         }
-        cplx<R> a = X[(k + 1) * N + k];
+        cplx<double> a = X[(k + 1) * N + k];
         if (a.re == 0 && a.im == 0) return {-INFINITY, 0};
         lre += 0.5 * ::log((double)a.re * a.re + (double)a.im * a.im);
         lim += ::atan2((double)a.im, (double)a.re);
         int n2 = N - k - 2;
         for (int e = lane; e < n2 * n2; e += 64) {
             int i = k + 2 + e % n2, j = k + 2 + e / n2;
-            cplx<R> up = X[i * N + k + 1] * X[j * N + k] - X[i * N + k] * X[j * N + k + 1];
+            cplx<double> up = X[i * N + k + 1] * X[j * N + k] - X[i * N + k] * X[j * N + k + 1];
             X[j * N + i] = X[j * N + i] + cdiv(up, a);
         }
         wave_sync();
