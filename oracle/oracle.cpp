@@ -1522,4 +1522,111 @@ int orc_exact_expectation(const orc_model* m, double* out) {
 
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 
+/* ---------------------------------------------------- learned proposal --
+ * No counterpart in the reference (its proposals are single and bond flips,
+ * metropolis_sampler.cpp:195-219).  This restates the build's own design:
+ *
+ * Cluster catalogue: the loops of the lattice (Kitaev: hexagons,
+ * honeycomb.cpp:201-209; toric: vertex stars, type 1 of
+ * toric_lattice.cpp:57-69).  For site i, cyc(i) is the first loop in list
+ * order that contains i, rotated to start at i; cluster (i, k) is its first
+ * k sites, k = 1 .. loop length. */
+int orc_clusters(const orc_model* m, int64_t* cyc, int64_t* kmax) {
+    GUARD({
+        const Model& md = m->s.model;
+        std::vector<std::vector<size_t>> loops;
+        if (md.cfg.model == "kitaev") {
+            loops = md.lat.hexagons();
+        } else {
+            for (auto& p : md.lat.plaqs())
+                if (p.second == 1) loops.push_back({p.first[0], p.first[1], p.first[2], p.first[3]});
+        }
+        size_t K = loops.at(0).size(), N = md.lat.n_total;
+        *kmax = (int64_t)K;
+        for (size_t i = 0; i < N; i++) {
+            bool found = false;
+            for (auto& l : loops) {
+                auto it = std::find(l.begin(), l.end(), i);
+                if (it == l.end()) continue;
+                size_t p0 = (size_t)(it - l.begin());
+                for (size_t t = 0; t < K; t++) cyc[i * K + t] = (int64_t)l[(p0 + t) % K];
+                found = true;
+                break;
+            }
+            if (!found) throw std::runtime_error("site on no loop");
+        }
+    })
+}
+
+/* Proposal network z(s) = W2 relu(W1 s + b1) + b2 with s_j = +-1 (bit set
+ * <=> -1), W1 row-major Hn x N, W2 row-major N x Hn; FP64. */
+static void mlp_fwd(int64_t N, int64_t Hn, const double* W1, const double* b1, const double* W2, const double* b2,
+                    const double* s, double* u, double* z) {
+    for (int64_t h = 0; h < Hn; h++) {
+        double acc = b1[h];
+        for (int64_t j = 0; j < N; j++) acc += W1[h * N + j] * s[j];
+        u[h] = acc;
+    }
+    for (int64_t o = 0; o < N; o++) {
+        double acc = b2[o];
+        for (int64_t h = 0; h < Hn; h++) acc += W2[o * Hn + h] * std::max(u[h], 0.);
+        z[o] = acc;
+    }
+}
+
+int orc_mlp_logits(int64_t N, int64_t Hn, const double* W1, const double* b1, const double* W2, const double* b2,
+                   const uint64_t* st, int64_t nw, int64_t n, double* z) {
+    GUARD({
+        std::vector<double> s(N), u(Hn);
+        for (int64_t r = 0; r < n; r++) {
+            for (int64_t j = 0; j < N; j++) s[j] = ((st[r * nw + (j >> 6)] >> (j & 63)) & 1) ? -1. : 1.;
+            mlp_fwd(N, Hn, W1, b1, W2, b2, s.data(), u.data(), z + r * N);
+        }
+    })
+}
+
+/* Batch loss L = (1/n) sum_r [ -w_r ln p_{i_r}(s_r) - beta H(p(s_r)) ],
+ * p = softmax(z), H = -sum p ln p, and its gradient by backpropagation:
+ * dL/dz_j = (1/n) [ w_r (p_j - [j = i_r]) + beta p_j (ln p_j + H) ]. */
+int orc_mlp_grad(int64_t N, int64_t Hn, const double* W1, const double* b1, const double* W2, const double* b2,
+                 const uint64_t* st, int64_t nw, int64_t n, const int64_t* seed, const double* w, double beta,
+                 double* gW1, double* gb1, double* gW2, double* gb2, double* loss) {
+    GUARD({
+        std::fill(gW1, gW1 + Hn * N, 0.);
+        std::fill(gb1, gb1 + Hn, 0.);
+        std::fill(gW2, gW2 + N * Hn, 0.);
+        std::fill(gb2, gb2 + N, 0.);
+        std::vector<double> s(N), u(Hn), z(N), p(N), dz(N), da(Hn);
+        double L = 0;
+        for (int64_t r = 0; r < n; r++) {
+            for (int64_t j = 0; j < N; j++) s[j] = ((st[r * nw + (j >> 6)] >> (j & 63)) & 1) ? -1. : 1.;
+            mlp_fwd(N, Hn, W1, b1, W2, b2, s.data(), u.data(), z.data());
+            double mx = *std::max_element(z.begin(), z.end()), zs = 0;
+            for (int64_t o = 0; o < N; o++) zs += std::exp(z[o] - mx);
+            double lse = mx + std::log(zs), H = 0;
+            for (int64_t o = 0; o < N; o++) {
+                p[o] = std::exp(z[o] - lse);
+                H -= p[o] * (z[o] - lse);
+            }
+            L += -w[r] * (z[seed[r]] - lse) - beta * H;
+            for (int64_t o = 0; o < N; o++)
+                dz[o] = (w[r] * (p[o] - (o == seed[r] ? 1. : 0.)) + beta * p[o] * ((z[o] - lse) + H)) / (double)n;
+            for (int64_t h = 0; h < Hn; h++) {
+                double a = std::max(u[h], 0.), acc = 0;
+                for (int64_t o = 0; o < N; o++) {
+                    gW2[o * Hn + h] += dz[o] * a;
+                    acc += W2[o * Hn + h] * dz[o];
+                }
+                da[h] = u[h] > 0 ? acc : 0.;
+            }
+            for (int64_t o = 0; o < N; o++) gb2[o] += dz[o];
+            for (int64_t h = 0; h < Hn; h++) {
+                gb1[h] += da[h];
+                for (int64_t j = 0; j < N; j++) gW1[h * N + j] += da[h] * s[j];
+            }
+        }
+        *loss = L / (double)n;
+    })
+}
+
 }  // extern "C"

@@ -117,6 +117,16 @@ int orc_exact_expectation(const orc_model* m, double* e_out /*2*/);
 /* Philox-4x32-10 block (for stream tests) */
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* learned proposal (no counterpart in the reference): cluster catalogue
+ * cyc (N x kmax) and the FP64 proposal network, logits and batch gradient */
+int orc_clusters(const orc_model* m, int64_t* cyc, int64_t* kmax);
+int orc_mlp_logits(int64_t N, int64_t Hn, const double* W1, const double* b1, const double* W2,
+                   const double* b2, const uint64_t* st, int64_t nw, int64_t n, double* z);
+int orc_mlp_grad(int64_t N, int64_t Hn, const double* W1, const double* b1, const double* W2,
+                 const double* b2, const uint64_t* st, int64_t nw, int64_t n, const int64_t* seed,
+                 const double* w, double beta, double* gW1, double* gb1, double* gW2, double* gb2,
+                 double* loss);
+
 #ifdef __cplusplus
 }
 #endif

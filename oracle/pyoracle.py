@@ -246,6 +246,46 @@ class Oracle:
         _chk(lib().orc_exact_expectation(self.h, _p(out, DP)))
         return complex(out[0], out[1])
 
+    def clusters(self):
+        """cluster catalogue of the learned proposal: cyc (N x kmax)"""
+        cyc = np.zeros(self.N * 8, dtype=np.int64)
+        k = C.c_int64()
+        _chk(lib().orc_clusters(self.h, _p(cyc, I64P), C.byref(k)))
+        return cyc[: self.N * k.value].reshape(self.N, k.value)
+
+
+def _mlp_args(W1, b1, W2, b2):
+    W1 = np.ascontiguousarray(W1, dtype=np.float64)
+    W2 = np.ascontiguousarray(W2, dtype=np.float64)
+    return (W1, np.ascontiguousarray(b1, dtype=np.float64), W2, np.ascontiguousarray(b2, dtype=np.float64))
+
+
+def mlp_logits(W1, b1, W2, b2, states, N):
+    """FP64 logits z (n x N) of the proposal network for packed states"""
+    W1, b1, W2, b2 = _mlp_args(W1, b1, W2, b2)
+    Hn = W1.shape[0]
+    st = np.ascontiguousarray(states, dtype=np.uint64).reshape(len(states), -1)
+    z = np.zeros((len(st), N))
+    _chk(lib().orc_mlp_logits(C.c_int64(N), C.c_int64(Hn), _p(W1, DP), _p(b1, DP), _p(W2, DP), _p(b2, DP),
+                              _p(st, U64P), C.c_int64(st.shape[1]), C.c_int64(len(st)), _p(z, DP)))
+    return z
+
+
+def mlp_grad(W1, b1, W2, b2, states, N, seeds, w, beta):
+    """loss and gradients (gW1, gb1, gW2, gb2) of the proposal-network
+    batch objective"""
+    W1, b1, W2, b2 = _mlp_args(W1, b1, W2, b2)
+    Hn = W1.shape[0]
+    st = np.ascontiguousarray(states, dtype=np.uint64).reshape(len(states), -1)
+    sd = np.ascontiguousarray(seeds, dtype=np.int64)
+    ww = np.ascontiguousarray(w, dtype=np.float64)
+    g = [np.zeros_like(W1), np.zeros_like(b1), np.zeros_like(W2), np.zeros_like(b2)]
+    loss = C.c_double()
+    _chk(lib().orc_mlp_grad(C.c_int64(N), C.c_int64(Hn), _p(W1, DP), _p(b1, DP), _p(W2, DP), _p(b2, DP),
+                            _p(st, U64P), C.c_int64(st.shape[1]), C.c_int64(len(st)), _p(sd, I64P), _p(ww, DP),
+                            C.c_double(beta), *[_p(x, DP) for x in g], C.byref(loss)))
+    return loss.value, g
+
 
 def apply_S(O, mean, var, r1, r2, rd, n_neural, x):
     P, S = O.shape
