@@ -82,12 +82,26 @@ struct M<float> {
 // th and sech^2 come from em = expm1(-2|x|): th = -em / (2 + em),
 // sech^2 = 4 (1 + em) / (2 + em)^2.
 //
-// FP32 evaluates the same formula from hardware transcendentals: expm1 by
-// its Taylor polynomial for |2x| < 1/4 (truncation < 5e-8 relative) and
-// exp - 1 beyond, sin/cos of the argument reduced to [-1/2, 1/2]
-// revolutions (v_sin_f32 / v_cos_f32), reciprocals by v_rcp_f32.  The
-// result carries a few 1e-7 relative error, like every FP32 quantity of
-// the sampler.
+// FP32 evaluates the same formula with cheaper pieces: expm1 by its Taylor
+// polynomial for |2x| < 1/4 (truncation < 5e-8 relative) and exp - 1
+// beyond, sin/cos by sincos_f below, reciprocals by v_rcp_f32 (1 ulp).
+//
+// sincos_f: y = n pi/2 + x with n = rint(2y/pi) and |x| <= pi/4 (two-term
+// Cody-Waite split of pi/2), then the Taylor series of sin to x^9 and of
+// cos to x^10 (truncation < 2e-9 on |x| <= pi/4) and the quadrant swap.
+__device__ inline void sincos_f(float y, float& s, float& c) {
+    float n = __builtin_rintf(y * 0.63661977236758134f);
+    float x = __builtin_fmaf(-n, 1.57079637050628662f, y);  // pi/2 rounded to float
+    x = __builtin_fmaf(n, 4.371139006309477e-8f, x);      // (that float) - pi/2
+    float x2 = x * x;
+    float sp = x * (1.f + x2 * (-1.f / 6 + x2 * (1.f / 120 + x2 * (-1.f / 5040 + x2 * (1.f / 362880)))));
+    float cp = 1.f + x2 * (-0.5f + x2 * (1.f / 24 + x2 * (-1.f / 720 + x2 * (1.f / 40320 + x2 * (-1.f / 3628800)))));
+    int q = (int)n & 3;
+    float ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
+    s = (q & 2) ? -ss : ss;
+    c = ((q + 1) & 2) ? -cc : cc;
+}
+
 __device__ inline cplx<float> ctanh_fast(cplx<float> z) {
     float ax = __builtin_fabsf(z.re), t = -2.f * ax;
     float ps = t * (1.f + t * (0.5f + t * (1.f / 6 + t * (1.f / 24 + t * (1.f / 120 + t * (1.f / 720))))));
@@ -95,9 +109,8 @@ __device__ inline cplx<float> ctanh_fast(cplx<float> z) {
     float q = __builtin_amdgcn_rcpf(2.f + em);
     float th = __builtin_copysignf(-em * q, z.re);
     float sech2 = 4.f * (1.f + em) * q * q;
-    float r = z.im * 0.15915494309189535f;  // revolutions
-    r -= __builtin_rintf(r);
-    float s = __builtin_amdgcn_sinf(r), c = __builtin_amdgcn_cosf(r);
+    float s, c;
+    sincos_f(z.im, s, c);
     float inv = __builtin_amdgcn_rcpf(c * c + th * th * s * s);
     return {th * inv, s * c * sech2 * inv};
 }
