@@ -122,11 +122,13 @@ def cpu_baseline():
                        f"sampling {st['t_sample']:.2f} s, SR {st['t_opt']:.2f} s")
 
 
-# HBM bytes per k_struct launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (KiB per dispatch) of this workload, scaled by the copy-kernel factor
-# (bytes a plain copy moves / what the counters report for it); None until
-# measured for the current kernel
-STRUCT_TRAFFIC_BYTES = None
+# HBM bytes per k_struct launch of this workload from two rocprofv3 --pmc
+# runs (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the CG
+# matvecs), each scaled by the factor a plain 1 GiB device copy gives
+# (bench.py --copy-calibration 1024: FETCH_SIZE reads 524302 KiB for
+# 1048576 KiB read -> x2.0; WRITE_SIZE reads 1048576 KiB -> x1.0):
+# 62826 KiB x 2.0 + 8068 KiB x 1.0
+STRUCT_TRAFFIC_BYTES = int((62826.4 * 1048576 / 524302.3 + 8067.8) * 1024)
 
 
 def sx_roofline(g, m, S, sx_ms):
@@ -186,7 +188,7 @@ def main():
     S = WORKLOAD["chains"] * WORKLOAD["samples_per_chain"]
     ms = 1e3 * dt / a.steps
     value = S * a.steps / dt
-    roofline = sx_roofline(g, m, S, times["sx"])
+    roofline = sx_roofline(g, m, S, times["sx_main"])
     last = log[-1]
     tau = float(np.mean([l["tau"] for l in log[a.warmup:]]))
     out = {

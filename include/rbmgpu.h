@@ -144,9 +144,10 @@ int rbmgpu_get_samples(const rbmgpu_ctx* ctx, uint64_t* states, double* obs,
 int rbmgpu_n_samples(const rbmgpu_ctx* ctx, int64_t* n);
 
 /* device time of the last launch of each kernel family, from HIP events
- * on the context's stream: [sample, obs/O assembly, reduce, S.x (mean per
- * CG matvec), cg vector update] in milliseconds. */
-int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms5);
+ * on the context's stream (no counterpart in the reference): [sample,
+ * obs/O assembly, reduce, S.x (mean per CG matvec), cg vector update, S.x
+ * main kernel alone (k_struct or k_sx_partial, mean per matvec)] in ms. */
+int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms6);
 int rbmgpu_synchronize(rbmgpu_ctx* ctx);
 
 #ifdef __cplusplus

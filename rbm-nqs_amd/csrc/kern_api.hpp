@@ -183,6 +183,8 @@ template <typename R>
 hipError_t launch_build_O(const OArgs<R>& a, hipStream_t st);
 template <typename R>
 hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st);
+// block partials of S.x over the materialised O (k_sx_partial); the caller
+// reduces them with launch_sx_reduce and finishes with launch_sx_finish
 template <typename R>
 hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st);
 // ZM (P <= 64*ZM) supported by launch_sx

@@ -92,12 +92,6 @@ static hipError_t sx_one(const SxArgs<R>& a, int blocks, hipStream_t st) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), lds, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_sx_reduce<R>), dim3((a.P + 255) / 256, a.nchunks), dim3(256), 0, st, a);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_sx_finish<R>), dim3((a.P + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 template <typename R>

@@ -78,9 +78,10 @@ struct rbmgpu_ctx {
     int block = 256;
     // timing
     hipEvent_t ev[12] = {};
-    double times[5] = {0, 0, 0, 0, 0};
+    double times[6] = {0, 0, 0, 0, 0, 0};
     std::vector<char> staging;
     hipEvent_t sxev[34] = {};
+    hipEvent_t sxmid[16] = {};  // after the main S.x kernel of each matvec of a batch
     std::string err;
 };
 
@@ -629,14 +630,18 @@ int ensure_sx(rbmgpu_ctx* c) {
     return 0;
 }
 
+int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st);
+
+// argument block of the S.x reduction / finish (and of the materialised-O
+// partial kernel)
 template <typename R>
-int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st) {
-    rbmgpu_ctx* ctx = c;
+SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd,
+                  const CgStatus* st) {
     SxArgs<R> a{};
     a.S = c->S;
     a.nblocks = c->sx_blocks;
     a.P = (int)c->P;
-    a.ld = (int)c->ld;
+    a.ld = (int)(c->structured ? c->ld_s : c->ld);
     a.nchunks = rbmgpu_ctx::kSxChunks;
     a.part2 = c->d_sxpart2;
     a.n_neural = (int)c->P_rbm;
@@ -651,7 +656,23 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
     a.part = (cplx<R>*)c->d_sxpart;
     a.y = y;
     a.status = st;
-    HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream));
+    return a;
+}
+
+// block partials (the dominant kernel), optional event, reduction, finish
+template <typename R>
+int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
+           hipEvent_t mid) {
+    rbmgpu_ctx* ctx = c;
+    SxArgs<R> a = sx_args<R>(c, x, y, r1, r2, rd, st);
+    if (c->structured) {
+        if (int rc = run_struct(c, 0, x, st)) return rc;
+    } else {
+        HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream));
+    }
+    if (mid) HIPCK(hipEventRecord(mid, c->stream));
+    HIPCK(launch_sx_reduce<R>(a, c->stream));
+    HIPCK(launch_sx_finish<R>(a, c->stream));
     return 0;
 }
 
@@ -676,34 +697,10 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st) {
     return 0;
 }
 
-int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st) {
-    if (c->structured) {
-        rbmgpu_ctx* ctx = c;
-        if (int rc = run_struct(c, 0, x, st)) return rc;
-        SxArgs<float> a{};
-        a.S = c->S;
-        a.nblocks = c->sx_blocks;
-        a.P = (int)c->P;
-        a.ld = (int)c->ld_s;
-        a.nchunks = rbmgpu_ctx::kSxChunks;
-        a.part = (cplx<float>*)c->d_sxpart;
-        a.part2 = c->d_sxpart2;
-        a.n_neural = (int)c->P_rbm;
-        a.x = x;
-        a.mean = c->d_omean;
-        a.var = c->d_ovar;
-        a.maxvar = c->d_maxvar;
-        a.r1 = r1;
-        a.r2 = r2;
-        a.rd = rd;
-        a.y = y;
-        a.status = st;
-        HIPCK(launch_sx_reduce<float>(a, c->stream));
-        HIPCK(launch_sx_finish<float>(a, c->stream));
-        return 0;
-    }
-    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st)
-                                  : run_sx<double>(c, x, y, r1, r2, rd, st);
+int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
+       hipEvent_t mid = nullptr) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, mid)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st, mid);
 }
 
 int compute_moments(rbmgpu_ctx* c) {
@@ -926,6 +923,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCK(hipEventCreate(&e));
     for (auto& e : c->sxev) HIPCK(hipEventCreate(&e));
+    for (auto& e : c->sxmid) HIPCK(hipEventCreate(&e));
     std::vector<uint8_t> perm8(c->perm.begin(), c->perm.end());
     if (upload(ctx, &c->d_perm, perm8.data(), perm8.size())) return bad(c->err);
     std::vector<int32_t> inv((size_t)c->nsymm * c->N), vbs((size_t)c->nsymm * c->nvb);
@@ -973,6 +971,8 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->sxev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->sxmid)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1130,7 +1130,7 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
     HIPCK(hipGetLastError());
     CgStatus st{};
     int64_t launched = 0;
-    double sx_ms = 0;
+    double sx_ms = 0, main_ms = 0;
     int64_t sx_n = 0;
     const int batch = 16;
     while (true) {
@@ -1138,7 +1138,7 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
         if (nb <= 0) break;
         for (int i = 0; i < nb; i++) {
             HIPCK(hipEventRecord(c->sxev[2 * i], c->stream));
-            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st)) return rc;
+            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->sxmid[i])) return rc;
             HIPCK(hipEventRecord(c->sxev[2 * i + 1], c->stream));
             hipLaunchKernelGGL(k_cg_step, dim3(1), dim3(1024), 0, c->stream, c->d_x, c->d_r, c->d_p, c->d_Ap,
                                (int)c->P, c->d_st);
@@ -1152,6 +1152,8 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
             float ms = 0;
             HIPCK(hipEventElapsedTime(&ms, c->sxev[2 * i], c->sxev[2 * i + 1]));
             sx_ms += ms;
+            HIPCK(hipEventElapsedTime(&ms, c->sxev[2 * i], c->sxmid[i]));
+            main_ms += ms;
             sx_n++;
         }
         launched += nb;
@@ -1161,7 +1163,10 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
     HIPCK(hipStreamSynchronize(c->stream));
     if (itn) *itn = st.itn;
     if (rs) *rs = st.rs;
-    if (sx_n) c->times[3] = sx_ms / sx_n;
+    if (sx_n) {
+        c->times[3] = sx_ms / sx_n;
+        c->times[5] = main_ms / sx_n;
+    }
     return 0;
 }
 
@@ -1228,9 +1233,9 @@ int rbmgpu_n_samples(const rbmgpu_ctx* c, int64_t* n) {
     return 0;
 }
 
-int rbmgpu_kernel_times(const rbmgpu_ctx* c, double* ms5) {
-    if (!c || !ms5) return -1;
-    for (int i = 0; i < 5; i++) ms5[i] = c->times[i];
+int rbmgpu_kernel_times(const rbmgpu_ctx* c, double* ms6) {
+    if (!c || !ms6) return -1;
+    for (int i = 0; i < 6; i++) ms6[i] = c->times[i];
     return 0;
 }
 

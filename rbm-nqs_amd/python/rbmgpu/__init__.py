@@ -279,9 +279,9 @@ class Gpu:
         return st, ob, Om, ca
 
     def kernel_times(self):
-        t = np.zeros(5)
+        t = np.zeros(6)
         self._chk(libs()[0].rbmgpu_kernel_times(self.c, _p(t, DP)))
-        return dict(zip(["sample", "assemble", "reduce", "sx", "cg"], t.tolist()))
+        return dict(zip(["sample", "assemble", "reduce", "sx", "cg", "sx_main"], t.tolist()))
 
     def synchronize(self):
         self._chk(libs()[0].rbmgpu_synchronize(self.c))
