@@ -142,7 +142,9 @@ struct StructArgs {
 };
 // LDS bytes of k_struct for padded sizes, 0 if no instance covers N, M
 size_t struct_lds_bytes(int N, int M);
-hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st);
+// e0 / e1 (optional): events recorded at the start and end of the dispatch
+hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr);
 template <typename R>
 hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st);
 template <typename R>
@@ -186,7 +188,8 @@ hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st);
 // block partials of S.x over the materialised O (k_sx_partial); the caller
 // reduces them with launch_sx_reduce and finishes with launch_sx_finish
 template <typename R>
-hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st);
+hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t e0 = nullptr,
+                     hipEvent_t e1 = nullptr);
 // ZM (P <= 64*ZM) supported by launch_sx
 int sx_max_params();
 

@@ -1,6 +1,8 @@
 // kern_launch.hpp -- launchers that dispatch the runtime launch shape to the
 // compiled template instances; included once per precision.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "kern_impl.hpp"
 
 namespace rbmgpu {
@@ -84,24 +86,27 @@ hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st) {
 }
 
 template <typename R, int ZM>
-static hipError_t sx_one(const SxArgs<R>& a, int blocks, hipStream_t st) {
+static hipError_t sx_one(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     size_t lds = 2 * (size_t)a.ld * sizeof(cplx<R>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)k_sx_partial<R, ZM>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), lds, st, a);
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), (uint32_t)lds, st, e0, e1, 0u, a);
+    else
+        hipLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st) {
-    if (a.ld <= 128 * 2) return sx_one<R, 2>(a, blocks, st);
-    if (a.ld <= 128 * 4) return sx_one<R, 4>(a, blocks, st);
-    if (a.ld <= 128 * 8) return sx_one<R, 8>(a, blocks, st);
-    if (a.ld <= 128 * 12) return sx_one<R, 12>(a, blocks, st);
-    if (a.ld <= 128 * 18) return sx_one<R, 18>(a, blocks, st);
-    if (a.ld <= 128 * 24) return sx_one<R, 24>(a, blocks, st);
+hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    if (a.ld <= 128 * 2) return sx_one<R, 2>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 4) return sx_one<R, 4>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 8) return sx_one<R, 8>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 12) return sx_one<R, 12>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 18) return sx_one<R, 18>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 24) return sx_one<R, 24>(a, blocks, st, e0, e1);
     return hipErrorInvalidValue;
 }
 
@@ -124,6 +129,6 @@ hipError_t launch_sx_finish(const SxArgs<R>& a, hipStream_t st) {
     template hipError_t launch_eval<R>(const EvalArgs<R>&, Shape, int, hipStream_t);      \
     template hipError_t launch_build_O<R>(const OArgs<R>&, hipStream_t);                  \
     template hipError_t launch_o_partial<R>(const MomArgs<R>&, int, hipStream_t);         \
-    template hipError_t launch_sx<R>(const SxArgs<R>&, int, hipStream_t);
+    template hipError_t launch_sx<R>(const SxArgs<R>&, int, hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace rbmgpu

@@ -1,4 +1,6 @@
 // FP32 MFMA instances of the structured (basic RBM) S.x / moments kernel.
+#include <hip/hip_ext.h>
+
 #include "kern_struct.hpp"
 
 namespace rbmgpu {
@@ -35,22 +37,26 @@ size_t struct_lds_bytes(int N, int M) {
 }
 
 template <int CT, int RT>
-static hipError_t one(const StructArgs& a, int blocks, hipStream_t st) {
+static hipError_t one(const StructArgs& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     size_t lds = lds_for(CT, RT);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)k_struct<CT, RT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_struct<CT, RT>), dim3(blocks), dim3(256), lds, st, a);
+    // with events: timestamps of this dispatch itself (start and end of the kernel)
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((k_struct<CT, RT>), dim3(blocks), dim3(256), (uint32_t)lds, st, e0, e1, 0u, a);
+    else
+        hipLaunchKernelGGL((k_struct<CT, RT>), dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st) {
+hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     int ct, rt;
     if (!pick(a.N, a.M, ct, rt)) return hipErrorInvalidValue;
 #define X(C_, R_) \
-    if (ct == C_ && rt == R_) return one<C_, R_>(a, blocks, st);
+    if (ct == C_ && rt == R_) return one<C_, R_>(a, blocks, st, e0, e1);
     STRUCT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

@@ -559,7 +559,8 @@ __global__ void __launch_bounds__(256) k_struct_moments(const double* A, const d
 }
 
 int ensure_sx(rbmgpu_ctx* c);
-int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st);
+int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hipEvent_t e0 = nullptr,
+               hipEvent_t e1 = nullptr);
 
 int struct_moments(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
@@ -630,8 +631,6 @@ int ensure_sx(rbmgpu_ctx* c) {
     return 0;
 }
 
-int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st);
-
 // argument block of the S.x reduction / finish (and of the materialised-O
 // partial kernel)
 template <typename R>
@@ -662,21 +661,20 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
 // block partials (the dominant kernel), optional event, reduction, finish
 template <typename R>
 int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-           hipEvent_t mid) {
+           hipEvent_t e0, hipEvent_t e1) {
     rbmgpu_ctx* ctx = c;
     SxArgs<R> a = sx_args<R>(c, x, y, r1, r2, rd, st);
     if (c->structured) {
-        if (int rc = run_struct(c, 0, x, st)) return rc;
+        if (int rc = run_struct(c, 0, x, st, e0, e1)) return rc;
     } else {
-        HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream));
+        HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream, e0, e1));
     }
-    if (mid) HIPCK(hipEventRecord(mid, c->stream));
     HIPCK(launch_sx_reduce<R>(a, c->stream));
     HIPCK(launch_sx_finish<R>(a, c->stream));
     return 0;
 }
 
-int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st) {
+int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hipEvent_t e0, hipEvent_t e1) {
     rbmgpu_ctx* ctx = c;
     StructArgs a{};
     a.S = c->S;
@@ -693,14 +691,15 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st) {
     a.x = x;
     a.part = (cplx<float>*)c->d_sxpart;
     a.status = st;
-    HIPCK(launch_struct(a, (int)c->sx_blocks, c->stream));
+    HIPCK(launch_struct(a, (int)c->sx_blocks, c->stream, e0, e1));
     return 0;
 }
 
+// e0 / e1: optional start / end events of the main kernel's dispatch
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-       hipEvent_t mid = nullptr) {
-    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, mid)
-                                  : run_sx<double>(c, x, y, r1, r2, rd, st, mid);
+       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st, e0, e1);
 }
 
 int compute_moments(rbmgpu_ctx* c) {
@@ -1137,8 +1136,8 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
         int nb = (int)std::min<int64_t>(batch, mi - launched);
         if (nb <= 0) break;
         for (int i = 0; i < nb; i++) {
-            HIPCK(hipEventRecord(c->sxev[2 * i], c->stream));
-            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->sxmid[i])) return rc;
+            // sxev[2i] / sxmid[i]: timestamps of the main kernel's dispatch
+            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->sxev[2 * i], c->sxmid[i])) return rc;
             HIPCK(hipEventRecord(c->sxev[2 * i + 1], c->stream));
             hipLaunchKernelGGL(k_cg_step, dim3(1), dim3(1024), 0, c->stream, c->d_x, c->d_r, c->d_p, c->d_Ap,
                                (int)c->P, c->d_st);
