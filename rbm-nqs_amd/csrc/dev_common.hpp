@@ -52,6 +52,8 @@ template <>
 struct M<double> {
     static __device__ double exp(double x) { return ::exp(x); }
     static __device__ double log(double x) { return ::log(x); }
+    static __device__ double log2(double x) { return ::log2(x); }
+    static __device__ double exp2(double x) { return ::exp2(x); }
     static __device__ double log1p(double x) { return ::log1p(x); }
     static __device__ double expm1(double x) { return ::expm1(x); }
     static __device__ double atan2(double y, double x) { return ::atan2(y, x); }
@@ -65,6 +67,9 @@ template <>
 struct M<float> {
     static __device__ float exp(float x) { return __expf(x); }
     static __device__ float log(float x) { return __logf(x); }
+    // v_log_f32 / v_exp_f32 (base 2, about 1 ulp)
+    static __device__ float log2(float x) { return __builtin_amdgcn_logf(x); }
+    static __device__ float exp2(float x) { return __builtin_amdgcn_exp2f(x); }
     static __device__ float log1p(float x) { return ::log1pf(x); }
     static __device__ float expm1(float x) { return ::expm1f(x); }
     static __device__ float atan2(float y, float x) { return ::atan2f(y, x); }
@@ -249,9 +254,12 @@ __device__ inline cplx<R> group_csum(cplx<R> v) {
     }
 }
 // product of complex numbers over the group with exponent tracking; the
-// result is returned as mantissa m and exponent e (value m * 2^e).  With
-// FMA contraction the lanes of a group may differ in the last bit of the
-// result, so callers use it as data only (never for control flow).
+// result is returned as mantissa m and exponent e (value m * 2^e).  Each
+// lane's mantissa is normalised once (max(|re|, |im|) in [0.5, 1), so |m|
+// in [0.5, sqrt 2)); the log2(L) <= 6 butterfly products then stay within
+// [2^-64, 2^32] and need no further scaling.  With FMA contraction the
+// lanes of a group may differ in the last bit of the result, so callers use
+// it as data only (never for control flow).
 template <int L, int O = 1, typename R>
 __device__ inline void group_cprod_steps(cplx<R>& m, int& e) {
     if constexpr (O < L) {
@@ -259,7 +267,6 @@ __device__ inline void group_cprod_steps(cplx<R>& m, int& e) {
         int qe = partner<O>(e);
         m = m * q;
         e += qe;
-        renorm(m, e);
         group_cprod_steps<L, 2 * O>(m, e);
     }
 }

@@ -25,15 +25,21 @@ namespace rbmgpu {
 
 __device__ inline int bitof(const uint64_t* st, int i) { return (int)((st[i >> 6] >> (i & 63)) & 1); }
 
+// Pointers into LDS are derived from the dynamic shared-memory base only
+// (never selected against a global pointer), so the compiler emits ds_read
+// for them; the tables that may live in either place have one pointer of
+// each kind and the code branches on the (uniform) staging flag.
 template <typename R>
 struct Lds {
     const cplx<R>* C;
     const cplx<R>* S;
-    const cplx<R>* W;
-    const uint8_t* P;
+    const cplx<R>* Wl;  // W in LDS (d.w_lds)
+    const cplx<R>* Wg;  // W in memory
+    const uint8_t* P;   // symmetry permutations (LDS; n_symm > 1)
     const cplx<R>* rb;  // ratio bias per site
     const int32_t* pb;  // proposal bonds (a, b)
-    Terms T;            // local operators (LDS copies when d.t_lds)
+    Terms Tl;           // local operators: LDS copies (d.t_lds)
+    Terms Tg;           //                  in memory
     PfLds<R> pf;        // Pfaffian tables and this wave's scratch (d.pf_rows > 0)
 };
 
@@ -55,7 +61,8 @@ __device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, const Terms& T, 
     Lds<R> l;
     l.C = lds_copy(at, t.C, nw);
     l.S = lds_copy(at, t.S, nw);
-    l.W = d.w_lds ? lds_copy(at, t.W, nw) : t.W;
+    l.Wg = t.W;
+    l.Wl = d.w_lds ? lds_copy(at, t.W, nw) : reinterpret_cast<const cplx<R>*>(at);
     l.rb = lds_copy(at, t.rb, d.N);
     l.pf = {};
     if (d.pf_rows) {
@@ -71,21 +78,22 @@ __device__ inline Lds<R> stage(const Dims& d, const Tabs<R>& t, const Terms& T, 
         l.pf.G = l.pf.D + 8 * 32;
         l.pf.F = reinterpret_cast<int*>(l.pf.G + 16 * 16);
     }
-    l.T = T;
+    l.Tg = T;
+    l.Tl = T;
     if (d.t_lds) {
-        l.T.val = lds_copy(at, T.val, 2 * (size_t)d.n_ent);
-        l.T.rowptr = lds_copy(at, T.rowptr, d.n_rowptr);
-        l.T.k = lds_copy(at, T.k, d.n_terms);
-        l.T.group = lds_copy(at, T.group, d.n_terms);
-        l.T.rowbase = lds_copy(at, T.rowbase, d.n_terms);
+        l.Tl.val = lds_copy(at, T.val, 2 * (size_t)d.n_ent);
+        l.Tl.rowptr = lds_copy(at, T.rowptr, d.n_rowptr);
+        l.Tl.k = lds_copy(at, T.k, d.n_terms);
+        l.Tl.group = lds_copy(at, T.group, d.n_terms);
+        l.Tl.rowbase = lds_copy(at, T.rowbase, d.n_terms);
     }
     l.pb = lds_copy(at, pb, 2 * (size_t)d.npb);
     if (d.pf_rows) l.pf.bs = lds_copy(at, t.bs, (size_t)d.N * d.N);
     if (d.t_lds) {
-        l.T.sites = lds_copy(at, T.sites, 8 * (size_t)d.n_terms);
-        l.T.mask = lds_copy(at, T.mask, d.n_ent);
+        l.Tl.sites = lds_copy(at, T.sites, 8 * (size_t)d.n_terms);
+        l.Tl.mask = lds_copy(at, T.mask, d.n_ent);
     }
-    l.P = d.n_symm > 1 ? lds_copy(at, t.perm, (size_t)d.n_symm * d.N) : t.perm;
+    l.P = d.n_symm > 1 ? lds_copy(at, t.perm, (size_t)d.n_symm * d.N) : reinterpret_cast<const uint8_t*>(at);
     __syncthreads();
     return l;
 }
@@ -115,7 +123,8 @@ struct Units {
             if (a[u] < 0) continue;
             cplx<R> acc = hb[a[u]];
             for (int f = 0; f < d.N; f++) {
-                cplx<R> w = l.W[kidx(d, l.P, s[u], f) * d.n_alpha + a[u]];
+                int wi = kidx(d, l.P, s[u], f) * d.n_alpha + a[u];
+                cplx<R> w = d.w_lds ? l.Wl[wi] : l.Wg[wi];
                 if (bitof(st, f)) acc = acc - w;
                 else acc = acc + w;
             }
@@ -171,13 +180,19 @@ struct Units {
     }
     // accepted move: theta -= 2 sigma_f W (rbm_base.cpp:174-192)
     __device__ void accept12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1) {
+        if (d.w_lds)
+            accept12_w(d, l, l.Wl, f0, s0, f1, s1);
+        else
+            accept12_w(d, l, l.Wg, f0, s0, f1, s1);
+    }
+    __device__ void accept12_w(const Dims& d, const Lds<R>& l, const cplx<R>* W, int f0, R s0, int f1, R s1) {
 #pragma unroll
         for (int u = 0; u < K; u++) {
             if (a[u] < 0) continue;
-            cplx<R> w0 = l.W[kidx(d, l.P, s[u], f0) * d.n_alpha + a[u]];
+            cplx<R> w0 = W[kidx(d, l.P, s[u], f0) * d.n_alpha + a[u]];
             th[u] = th[u] - (2 * s0) * w0;
             if (f1 >= 0) {
-                cplx<R> w1 = l.W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
+                cplx<R> w1 = W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
                 th[u] = th[u] - (2 * s1) * w1;
             }
             tn[u] = ctanh(th[u]);
@@ -213,10 +228,10 @@ __device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const 
 
 // local observables (local_op.cpp:31-60, summed per group as
 // local_op_chain.cpp:54-60); lane 0 writes out[2*g], out[2*g+1]
-template <typename R, int L, int K, bool PF>
-__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
-                                const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
-    const Terms& T = l.T;
+template <typename R, int L, int K, bool PF, bool TL>
+__device__ inline void eval_obs_t(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
+                                  const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
+    const Terms& T = TL ? l.Tl : l.Tg;
     cplx<double> acc{0, 0};
     int g = T.n_terms > 0 ? T.group[0] : 0;
     for (int ti = 0; ti < T.n_terms; ti++) {
@@ -256,6 +271,14 @@ __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L
         out[2 * g] = acc.re;
         out[2 * g + 1] = acc.im;
     }
+}
+template <typename R, int L, int K, bool PF>
+__device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
+                                const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
+    if (d.t_lds)
+        eval_obs_t<R, L, K, PF, true>(d, l, U, pv, st, ln, out);
+    else
+        eval_obs_t<R, L, K, PF, false>(d, l, U, pv, st, ln, out);
 }
 
 // ------------------------------------------------------------ sampler --
@@ -335,17 +358,19 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
             cplx<R> p;
             int pe;
             U.ratio12(d, l, f0, s0, f1, s1, p, pe);
-            R lp = M<R>::log(norm2(p)) + (R)(2 * pe) * (R)0.69314718055994530942;
-            R lnr2 = group_sum<L>(lp);
+            // log2 |psi'/psi|^2: group sum of the lanes' log2 |p|^2, plus the
+            // visible-bias term 2 Re(-2 sum sigma_f c_f) / ln 2
+            R lp = M<R>::log2(norm2(p)) + (R)(2 * pe);
+            R lr2 = group_sum<L>(lp);
             R bre = (-2 * s0) * l.rb[f0].re;
             if (f1 >= 0) bre += (-2 * s1) * l.rb[f1].re;
-            lnr2 += 2 * bre;
+            lr2 += (R)2.8853900817779268 * bre;  // 2 / ln 2
             if constexpr (PF) {
                 int fl[2] = {f0, f1};
                 cplx<R> q = pf_ratio<R, true>(l.pf, pv, st, fl, f1 >= 0 ? 3 : 1, ln, pp);
-                lnr2 += M<R>::log(norm2(q));
+                lr2 += M<R>::log2(norm2(q));
             }
-            if (uu < M<R>::exp(lnr2)) {
+            if (uu < M<R>::exp2(lr2)) {
                 U.accept12(d, l, f0, s0, f1, s1);
                 if constexpr (PF) pf_accept(l.pf, pv, pp, ln);
                 st[f0 >> 6] ^= (uint64_t)1 << (f0 & 63);
