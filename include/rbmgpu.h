@@ -145,8 +145,9 @@ int rbmgpu_n_samples(const rbmgpu_ctx* ctx, int64_t* n);
 
 /* device time of the last launch of each kernel family, from HIP events
  * on the context's stream (no counterpart in the reference): [sample,
- * obs/O assembly, reduce, S.x (mean per CG matvec), cg vector update, S.x
- * main kernel alone (k_struct or k_sx_partial, mean per matvec)] in ms. */
+ * obs/O assembly, reduce, S.x main kernel + chunk reduction (mean per CG
+ * matvec), unused (0), S.x main kernel alone (k_struct or k_sx_partial,
+ * mean per matvec)] in ms. */
 int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms6);
 int rbmgpu_synchronize(rbmgpu_ctx* ctx);
 
