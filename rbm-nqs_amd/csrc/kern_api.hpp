@@ -126,6 +126,7 @@ struct SxArgs {
     cplx<R>* part;       // nblocks x ld
     double* part2;       // nchunks x P complex
     double* y;           // complex P
+    double* pdot;        // per column block: p^H y partial (2 doubles), optional
     const CgStatus* status;
 };
 
@@ -147,8 +148,11 @@ hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st, hipEve
                          hipEvent_t e1 = nullptr);
 template <typename R>
 hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st);
+// column sums of the block partials and the finished product y = S x in one
+// pass (k_sx_colsum), ceil(P / kColBlock) blocks (pdot entries)
 template <typename R>
-hipError_t launch_sx_finish(const SxArgs<R>& a, hipStream_t st);
+hipError_t launch_sx_colsum(const SxArgs<R>& a, hipStream_t st);
+constexpr int kColBlock = 64;  // columns per k_sx_colsum block
 
 // supported (lanes per chain, units per lane) shapes
 struct Shape {
@@ -186,7 +190,7 @@ hipError_t launch_build_O(const OArgs<R>& a, hipStream_t st);
 template <typename R>
 hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st);
 // block partials of S.x over the materialised O (k_sx_partial); the caller
-// reduces them with launch_sx_reduce and finishes with launch_sx_finish
+// finishes the product with launch_sx_colsum
 template <typename R>
 hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t e0 = nullptr,
                      hipEvent_t e1 = nullptr);

@@ -539,8 +539,8 @@ __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
 // whole rows k: y_k = O_k . x by a wave reduction, then conj(O_k) y_k is
 // accumulated in lane registers (FP32: the row stays in registers; FP64:
 // the row is re-read from cache).  The four waves of a block add their
-// partials in LDS; the block partials are reduced by k_sx_reduce and
-// finished by k_sx_finish (regularisation and the <O> term).
+// partials in LDS; k_sx_colsum sums the block partials and finishes the
+// product (regularisation and the <O> term).
 template <typename R>
 __device__ inline void load_pair(const cplx<R>* o, int p, cplx<R>& a, cplx<R>& b);
 template <>
@@ -650,11 +650,40 @@ __global__ void __launch_bounds__(256) k_sx_reduce(SxArgs<R> a) {
     a.part2[2 * (blockIdx.y * (int64_t)a.P + p) + 1] = zi;
 }
 
+// y = S x from the block partials in one pass: block b owns columns
+// 64 b .. 64 b + 63 (lane = column, consecutive addresses); its four waves
+// sum disjoint quarters of the nblocks partial rows, combine them in LDS,
+// and wave 0 finishes y_c = z_c / S + reg_c x_c - conj(mu_c) (mu^T x)
+// (cg_solver.cpp:301-317).  With pdot, each block also writes its partial
+// of x^H y for the CG step.
 template <typename R>
-__global__ void __launch_bounds__(256) k_sx_finish(SxArgs<R> a) {
+__global__ void __launch_bounds__(256) k_sx_colsum(SxArgs<R> a) {
     if (a.status && a.status->done) return;
+    __shared__ double zs[4][kColBlock][2];
     __shared__ double red[2][256];
-    // mu^T x (computed by every block; P is small)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = blockIdx.x * kColBlock + lane;
+    // quarter of the partial rows
+    const int64_t per = (a.nblocks + 3) / 4, b0 = w * per;
+    const int64_t b1 = b0 + per < a.nblocks ? b0 + per : a.nblocks;
+    double zr = 0, zi = 0;
+    if (c < a.P) {
+        int64_t b = b0;
+        for (; b + 4 <= b1; b += 4) {
+            cplx<R> v0 = a.part[b * a.ld + c], v1 = a.part[(b + 1) * a.ld + c];
+            cplx<R> v2 = a.part[(b + 2) * a.ld + c], v3 = a.part[(b + 3) * a.ld + c];
+            zr += ((double)v0.re + (double)v1.re) + ((double)v2.re + (double)v3.re);
+            zi += ((double)v0.im + (double)v1.im) + ((double)v2.im + (double)v3.im);
+        }
+        for (; b < b1; b++) {
+            cplx<R> v = a.part[b * a.ld + c];
+            zr += v.re;
+            zi += v.im;
+        }
+    }
+    zs[w][lane][0] = zr;
+    zs[w][lane][1] = zi;
+    // mu^T x (all threads, P is small)
     double mr = 0, mi = 0;
     for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
         double ur = a.mean[2 * p], ui = a.mean[2 * p + 1], xr = a.x[2 * p], xi = a.x[2 * p + 1];
@@ -671,21 +700,32 @@ __global__ void __launch_bounds__(256) k_sx_finish(SxArgs<R> a) {
         }
         __syncthreads();
     }
+    if (w != 0) return;
     mr = red[0][0];
     mi = red[1][0];
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
-    double zr = 0, zi = 0;
-    for (int c = 0; c < a.nchunks; c++) {
-        zr += a.part2[2 * ((int64_t)c * a.P + p)];
-        zi += a.part2[2 * ((int64_t)c * a.P + p) + 1];
+    double dr = 0, di = 0;
+    if (c < a.P) {
+        zr = (zs[0][lane][0] + zs[1][lane][0]) + (zs[2][lane][0] + zs[3][lane][0]);
+        zi = (zs[0][lane][1] + zs[1][lane][1]) + (zs[2][lane][1] + zs[3][lane][1]);
+        double inv = 1.0 / (double)a.S;
+        double reg = a.var[c] * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
+        double xr = a.x[2 * c], xi = a.x[2 * c + 1];
+        double ur = a.mean[2 * c], ui = -a.mean[2 * c + 1];  // conj(mu_c)
+        double yr = zr * inv + reg * xr - (ur * mr - ui * mi);
+        double yi = zi * inv + reg * xi - (ur * mi + ui * mr);
+        a.y[2 * c] = yr;
+        a.y[2 * c + 1] = yi;
+        dr = xr * yr + xi * yi;  // conj(x) y
+        di = xr * yi - xi * yr;
     }
-    double inv = 1.0 / (double)a.S;
-    double reg = a.var[p] * (p < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
-    double xr = a.x[2 * p], xi = a.x[2 * p + 1];
-    double ur = a.mean[2 * p], ui = -a.mean[2 * p + 1];  // conj(mu_p)
-    a.y[2 * p] = zr * inv + reg * xr - (ur * mr - ui * mi);
-    a.y[2 * p + 1] = zi * inv + reg * xi - (ur * mi + ui * mr);
+    if (a.pdot) {
+        dr = group_sum<64>(dr);
+        di = group_sum<64>(di);
+        if (lane == 0) {
+            a.pdot[2 * blockIdx.x] = dr;
+            a.pdot[2 * blockIdx.x + 1] = di;
+        }
+    }
 }
 
 }  // namespace rbmgpu

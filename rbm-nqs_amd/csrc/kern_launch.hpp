@@ -117,14 +117,14 @@ hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st) {
 }
 
 template <typename R>
-hipError_t launch_sx_finish(const SxArgs<R>& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_sx_finish<R>), dim3((a.P + 255) / 256), dim3(256), 0, st, a);
+hipError_t launch_sx_colsum(const SxArgs<R>& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_sx_colsum<R>), dim3((a.P + kColBlock - 1) / kColBlock), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 #define RBM_INSTANTIATE(R)                                                                  \
+    template hipError_t launch_sx_colsum<R>(const SxArgs<R>&, hipStream_t);               \
     template hipError_t launch_sx_reduce<R>(const SxArgs<R>&, hipStream_t);               \
-    template hipError_t launch_sx_finish<R>(const SxArgs<R>&, hipStream_t);               \
     template hipError_t launch_sample<R>(const SampArgs<R>&, Shape, int, hipStream_t);    \
     template hipError_t launch_eval<R>(const EvalArgs<R>&, Shape, int, hipStream_t);      \
     template hipError_t launch_build_O<R>(const OArgs<R>&, hipStream_t);                  \

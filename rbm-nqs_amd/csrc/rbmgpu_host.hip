@@ -63,13 +63,12 @@ struct rbmgpu_ctx {
     int64_t part_cap = 0;
     bool moments_valid = false;
     // CG / S.x
-    double *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr;
+    double *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_pdot = nullptr;
     CgStatus* d_st = nullptr;
     void* d_sxpart = nullptr;
     double* d_sxpart2 = nullptr;
     int64_t sx_blocks = 0, sxpart_cap = 0;
     static constexpr int kSxChunks = 32;  // chunk partials of the moment sums
-    static constexpr int kCgChunks = 8;   // chunk partials of S.x
     // eval scratch
     uint64_t* d_evst = nullptr;
     double *d_evout = nullptr, *d_evobs = nullptr;
@@ -469,39 +468,15 @@ __global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double*
     }
 }
 
-// One CG iteration fused with the end of its S.x product: y = S p from the
-// chunk partials of sum_k conj(O_k) (O_k . p) (k_sx_reduce), the <O> term
-// and the regularisation (cg_solver.cpp:301-317), then the CG update of
-// cg_solver.cpp:160-189.  One block; y is kept in Ap.
-__global__ void __launch_bounds__(1024) k_cg_fused(const double* part2, int nch, int64_t S, const double* mean,
-                                                   const double* var, const double* maxvar, double r1, double r2,
-                                                   double rd, int n_neural, double* x, double* r, double* p,
-                                                   double* Ap, int P, CgStatus* st) {
+// CG update after y = S p (cg_solver.cpp:160-189): p^H y from the column
+// blocks' partials (k_sx_colsum), then x, r, |r|^2 and p.  One block.
+__global__ void __launch_bounds__(1024) k_cg_update(const double* pdot, int nb, double* x, double* r, double* p,
+                                                    const double* Ap, int P, CgStatus* st) {
     if (st->done) return;
-    double mr = 0, mi = 0;  // mu^T p
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        double ur = mean[2 * i], ui = mean[2 * i + 1], pr = p[2 * i], pi = p[2 * i + 1];
-        mr += ur * pr - ui * pi;
-        mi += ur * pi + ui * pr;
-    }
-    block_sum2(mr, mi);
-    const double inv = 1.0 / (double)S, mv = *maxvar;
     double ar = 0, ai = 0;  // p^H Ap
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        double zr = 0, zi = 0;
-        for (int c = 0; c < nch; c++) {
-            zr += part2[2 * ((int64_t)c * P + i)];
-            zi += part2[2 * ((int64_t)c * P + i) + 1];
-        }
-        double reg = var[i] * (i < n_neural ? r1 : r1 + rd) + mv * r2;
-        double pr = p[2 * i], pi = p[2 * i + 1];
-        double ur = mean[2 * i], ui = -mean[2 * i + 1];  // conj(mu_i)
-        double yr = zr * inv + reg * pr - (ur * mr - ui * mi);
-        double yi = zi * inv + reg * pi - (ur * mi + ui * mr);
-        Ap[2 * i] = yr;
-        Ap[2 * i + 1] = yi;
-        ar += pr * yr + pi * yi;
-        ai += pr * yi - pi * yr;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        ar += pdot[2 * b];
+        ai += pdot[2 * b + 1];
     }
     block_sum2(ar, ai);
     double rsold = st->rsold;
@@ -626,6 +601,7 @@ int ensure_cg(rbmgpu_ctx* c) {
     if (int rc = dalloc(ctx, &c->d_r, vb)) return rc;
     if (int rc = dalloc(ctx, &c->d_p, vb)) return rc;
     if (int rc = dalloc(ctx, &c->d_Ap, vb)) return rc;
+    if (int rc = dalloc(ctx, &c->d_pdot, ((c->P + kColBlock - 1) / kColBlock) * 2 * sizeof(double))) return rc;
     if (int rc = dalloc(ctx, &c->d_st, sizeof(CgStatus))) return rc;
     return 0;
 }
@@ -667,7 +643,7 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
     a.nblocks = c->sx_blocks;
     a.P = (int)c->P;
     a.ld = (int)(c->structured ? c->ld_s : c->ld);
-    a.nchunks = rbmgpu_ctx::kCgChunks;
+    a.nchunks = rbmgpu_ctx::kSxChunks;
     a.part2 = c->d_sxpart2;
     a.n_neural = (int)c->P_rbm;
     a.O = (const cplx<R>*)c->d_O;
@@ -687,7 +663,7 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
 // block partials (the dominant kernel), optional event, reduction, finish
 template <typename R>
 int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-           hipEvent_t e0, hipEvent_t e1, bool finish) {
+           hipEvent_t e0, hipEvent_t e1, double* pdot) {
     rbmgpu_ctx* ctx = c;
     SxArgs<R> a = sx_args<R>(c, x, y, r1, r2, rd, st);
     if (c->structured) {
@@ -695,8 +671,8 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
     } else {
         HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream, e0, e1));
     }
-    HIPCK(launch_sx_reduce<R>(a, c->stream));
-    if (finish) HIPCK(launch_sx_finish<R>(a, c->stream));
+    a.pdot = pdot;
+    HIPCK(launch_sx_colsum<R>(a, c->stream));
     return 0;
 }
 
@@ -722,12 +698,11 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hip
 }
 
 // e0 / e1: optional start / end events of the main kernel's dispatch;
-// finish = false stops after the chunk partials (the fused CG step ends the
-// product)
+// pdot: optional per-column-block partials of x^H y for the CG update
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool finish = true) {
-    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1, finish)
-                                  : run_sx<double>(c, x, y, r1, r2, rd, st, e0, e1, finish);
+       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, double* pdot = nullptr) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1, pdot)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st, e0, e1, pdot);
 }
 
 int compute_moments(rbmgpu_ctx* c) {
@@ -1168,13 +1143,13 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
         if (nb <= 0) break;
         for (int i = 0; i < nb; i++) {
             // sxev[2i] / sxmid[i]: timestamps of the main kernel's dispatch;
-            // sxev[2i+1]: after the chunk reduction
-            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->sxev[2 * i], c->sxmid[i], false))
+            // sxev[2i+1]: after y = S p
+            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->sxev[2 * i], c->sxmid[i], c->d_pdot))
                 return rc;
             HIPCK(hipEventRecord(c->sxev[2 * i + 1], c->stream));
-            hipLaunchKernelGGL(k_cg_fused, dim3(1), dim3(1024), 0, c->stream, c->d_sxpart2,
-                               (int)rbmgpu_ctx::kCgChunks, c->S, c->d_omean, c->d_ovar, c->d_maxvar, r1, r2, rd,
-                               (int)c->P_rbm, c->d_x, c->d_r, c->d_p, c->d_Ap, (int)c->P, c->d_st);
+            hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, c->d_pdot,
+                               (int)((c->P + kColBlock - 1) / kColBlock), c->d_x, c->d_r, c->d_p, c->d_Ap,
+                               (int)c->P, c->d_st);
             HIPCK(hipGetLastError());
         }
         HIPCK(hipMemcpyAsync(&st, c->d_st, sizeof(CgStatus), hipMemcpyDeviceToHost, c->stream));
