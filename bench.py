@@ -188,7 +188,11 @@ def main():
     S = WORKLOAD["chains"] * WORKLOAD["samples_per_chain"]
     ms = 1e3 * dt / a.steps
     value = S * a.steps / dt
-    roofline = sx_roofline(g, m, S, times["sx_main"])
+    # dominant kernel alone: back-to-back launches on the last sample set,
+    # between two HIP events on the context's stream
+    sx_ms = g.time_sx(20)
+    roofline = sx_roofline(g, m, S, sx_ms)
+    roofline["launch_ms_in_cg"] = times["sx_main"]
     last = log[-1]
     tau = float(np.mean([l["tau"] for l in log[a.warmup:]]))
     out = {

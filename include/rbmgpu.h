@@ -150,6 +150,10 @@ int rbmgpu_n_samples(const rbmgpu_ctx* ctx, int64_t* n);
  * mean per matvec)] in ms. */
 int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms6);
 int rbmgpu_synchronize(rbmgpu_ctx* ctx);
+/* mean device time of the main S.x kernel (k_struct or k_sx_partial) over
+ * reps back-to-back launches on the current sample set, bracketed by two
+ * HIP events (no counterpart in the reference; measurement hook) */
+int rbmgpu_time_sx(rbmgpu_ctx* ctx, int32_t reps, double* ms);
 
 #ifdef __cplusplus
 }

@@ -651,20 +651,20 @@ __global__ void __launch_bounds__(256) k_sx_reduce(SxArgs<R> a) {
 }
 
 // y = S x from the block partials in one pass: block b owns columns
-// 64 b .. 64 b + 63 (lane = column, consecutive addresses); its four waves
-// sum disjoint quarters of the nblocks partial rows, combine them in LDS,
+// 64 b .. 64 b + 63 (lane = column, consecutive addresses); its 16 waves
+// sum disjoint slices of the nblocks partial rows, combine them in LDS,
 // and wave 0 finishes y_c = z_c / S + reg_c x_c - conj(mu_c) (mu^T x)
 // (cg_solver.cpp:301-317).  With pdot, each block also writes its partial
 // of x^H y for the CG step.
+constexpr int kColWaves = 16;
 template <typename R>
-__global__ void __launch_bounds__(256) k_sx_colsum(SxArgs<R> a) {
+__global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     if (a.status && a.status->done) return;
-    __shared__ double zs[4][kColBlock][2];
-    __shared__ double red[2][256];
+    __shared__ double zs[kColWaves][kColBlock][2];
+    __shared__ double red[2][64 * kColWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c = blockIdx.x * kColBlock + lane;
-    // quarter of the partial rows
-    const int64_t per = (a.nblocks + 3) / 4, b0 = w * per;
+    const int64_t per = (a.nblocks + kColWaves - 1) / kColWaves, b0 = w * per;
     const int64_t b1 = b0 + per < a.nblocks ? b0 + per : a.nblocks;
     double zr = 0, zi = 0;
     if (c < a.P) {
@@ -690,23 +690,28 @@ __global__ void __launch_bounds__(256) k_sx_colsum(SxArgs<R> a) {
         mr += ur * xr - ui * xi;
         mi += ur * xi + ui * xr;
     }
-    red[0][threadIdx.x] = mr;
-    red[1][threadIdx.x] = mi;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] += red[1][threadIdx.x + s];
-        }
-        __syncthreads();
+    mr = group_sum<64>(mr);
+    mi = group_sum<64>(mi);
+    if (lane == 0) {
+        red[0][w] = mr;
+        red[1][w] = mi;
     }
+    __syncthreads();
     if (w != 0) return;
-    mr = red[0][0];
-    mi = red[1][0];
+    mr = 0;
+    mi = 0;
+    for (int q = 0; q < kColWaves; q++) {
+        mr += red[0][q];
+        mi += red[1][q];
+    }
     double dr = 0, di = 0;
     if (c < a.P) {
-        zr = (zs[0][lane][0] + zs[1][lane][0]) + (zs[2][lane][0] + zs[3][lane][0]);
-        zi = (zs[0][lane][1] + zs[1][lane][1]) + (zs[2][lane][1] + zs[3][lane][1]);
+        zr = 0;
+        zi = 0;
+        for (int q = 0; q < kColWaves; q++) {
+            zr += zs[q][lane][0];
+            zi += zs[q][lane][1];
+        }
         double inv = 1.0 / (double)a.S;
         double reg = a.var[c] * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
         double xr = a.x[2 * c], xi = a.x[2 * c + 1];

@@ -118,7 +118,7 @@ hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st) {
 
 template <typename R>
 hipError_t launch_sx_colsum(const SxArgs<R>& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_sx_colsum<R>), dim3((a.P + kColBlock - 1) / kColBlock), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_sx_colsum<R>), dim3((a.P + kColBlock - 1) / kColBlock), dim3(64 * kColWaves), 0, st, a);
     return hipGetLastError();
 }
 

@@ -1115,6 +1115,32 @@ int rbmgpu_apply_S(rbmgpu_ctx* c, const double* x, double* y, double r1, double 
     return 0;
 }
 
+int rbmgpu_time_sx(rbmgpu_ctx* c, int32_t reps, double* ms) {
+    if (!c || !ms || reps < 1) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->moments_valid)
+        if (int rc = compute_moments(c)) return rc;
+    if (int rc = ensure_cg(c)) return rc;
+    if (int rc = ensure_sx(c)) return rc;
+    // the last CG direction (or zeros) as the operand; only the main kernel runs
+    HIPCK(hipEventRecord(c->ev[8], c->stream));
+    for (int i = 0; i < reps; i++) {
+        if (c->structured) {
+            if (int rc = run_struct(c, 0, c->d_p, nullptr)) return rc;
+        } else if (c->prec == RBMGPU_FP32) {
+            SxArgs<float> a = sx_args<float>(c, c->d_p, c->d_Ap, 0, 0, 0, nullptr);
+            HIPCK(launch_sx<float>(a, (int)c->sx_blocks, c->stream));
+        } else {
+            SxArgs<double> a = sx_args<double>(c, c->d_p, c->d_Ap, 0, 0, 0, nullptr);
+            HIPCK(launch_sx<double>(a, (int)c->sx_blocks, c->stream));
+        }
+    }
+    HIPCK(hipEventRecord(c->ev[9], c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    *ms = elapsed(c, 8, 9) / reps;
+    return 0;
+}
+
 int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double r2, double rd,
                     int64_t max_iter, double rtol, int64_t* itn, double* rs) {
     if (!c || !b || !x) return -1;

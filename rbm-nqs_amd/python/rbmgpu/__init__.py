@@ -55,7 +55,7 @@ def exported_symbols():
     """Names the two libraries must export (checked by the CPU tests)."""
     return {
         "gpu": ["rbmgpu_create", "rbmgpu_destroy", "rbmgpu_last_error", "rbmgpu_n_params",
-                "rbmgpu_launch_shape", "rbmgpu_set_params", "rbmgpu_get_params",
+                "rbmgpu_launch_shape", "rbmgpu_set_params", "rbmgpu_get_params", "rbmgpu_time_sx",
                 "rbmgpu_update_weights", "rbmgpu_sample_epoch", "rbmgpu_load_samples",
                 "rbmgpu_reduce", "rbmgpu_sr_solve", "rbmgpu_apply_S", "rbmgpu_eval_logpsi",
                 "rbmgpu_psi_ratio", "rbmgpu_get_samples", "rbmgpu_n_samples",
@@ -282,6 +282,12 @@ class Gpu:
         t = np.zeros(6)
         self._chk(libs()[0].rbmgpu_kernel_times(self.c, _p(t, DP)))
         return dict(zip(["sample", "assemble", "reduce", "sx", "cg", "sx_main"], t.tolist()))
+
+    def time_sx(self, reps=20):
+        """mean ms of the main S.x kernel over reps back-to-back launches"""
+        ms = C.c_double()
+        self._chk(libs()[0].rbmgpu_time_sx(self.c, C.c_int32(reps), C.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         self._chk(libs()[0].rbmgpu_synchronize(self.c))
