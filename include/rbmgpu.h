@@ -72,6 +72,15 @@ typedef struct rbmgpu_config {
                                 S.x (O never formed, rbmgpu_get_samples
                                 cannot return O); other machines
                                 materialise O regardless */
+    /* ---- learned proposal (no counterpart in the reference): cluster
+     * flips drawn by a network z(s) = W2 relu(W1 s + b1) + b2 under
+     * Metropolis-Hastings; FP32 machines without the Pfaffian factor */
+    int32_t learn_hidden;     /* hidden width, multiple of 16, <= 512; 0: off */
+    double learn_fraction;    /* probability of a learned move per slot */
+    int32_t cluster_kmax;     /* cluster sizes 1 .. kmax (<= 8) */
+    const int32_t* clusters;  /* N x kmax: the lattice loop through site i,
+                                 starting at i */
+    uint64_t learn_seed;      /* initial network weights */
 } rbmgpu_config;
 
 typedef struct rbmgpu_moments {
@@ -150,6 +159,26 @@ int rbmgpu_n_samples(const rbmgpu_ctx* ctx, int64_t* n);
  * mean per matvec)] in ms. */
 int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms6);
 int rbmgpu_synchronize(rbmgpu_ctx* ctx);
+/* ---- learned proposal (no counterpart in the reference) */
+/* one training step of the proposal network on the records of the last
+ * rbmgpu_sample_epoch (the last learned proposal of each chain before each
+ * kept sample): REINFORCE with reward a * k (acceptance probability times
+ * cluster size, centred on the batch mean) and entropy weight beta, Adam
+ * with learning rate lr on at most batch records.  out4: [loss, mean
+ * reward, learned moves proposed, accepted] of the epoch. */
+int rbmgpu_learn_train(rbmgpu_ctx* ctx, double lr, double beta, int64_t batch, double* out4);
+/* batch objective and gradient at the current weights for host-given
+ * records (parity hook): grads = [W1 (Hn x N) | b1 (Hn) | W2 (N x Hn) |
+ * b2 (N)] */
+int rbmgpu_learn_grad(rbmgpu_ctx* ctx, const uint64_t* states, const int32_t* seeds, const double* w,
+                      int64_t n, double beta, double* grads, double* loss);
+/* FP32 master weights in the same layout; set = 1 uploads them */
+int rbmgpu_learn_weights(rbmgpu_ctx* ctx, int32_t set, double* params);
+/* logits of host-given states through the sampler's bf16 MFMA forward */
+int rbmgpu_learn_logits(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n, double* z);
+/* learn_fraction for the following epochs (0: plain Metropolis) */
+int rbmgpu_learn_set_fraction(rbmgpu_ctx* ctx, double fraction);
+
 /* mean device time of the main S.x kernel (k_struct or k_sx_partial) over
  * reps back-to-back launches on the current sample set, bracketed by two
  * HIP events (no counterpart in the reference; measurement hook) */

@@ -22,10 +22,16 @@ rbmhost_model* rbmhost_create(const char* ini_text, char* err, int32_t err_len);
 void rbmhost_destroy(rbmhost_model* m);
 
 /* [N, P, n_neural, n_vb, n_alpha, n_symm, pf_rows, n_terms, n_groups,
- *  n_bonds, n_chains, n_samples, n_warmup, step_size, n_prop_bonds] */
-int rbmhost_info(const rbmhost_model* m, int64_t* out15);
+ *  n_bonds, n_chains, n_samples, n_warmup, step_size, n_prop_bonds,
+ *  learn.hidden, cluster kmax] */
+int rbmhost_info(const rbmhost_model* m, int64_t* out17);
 int rbmhost_bonds(const rbmhost_model* m, int64_t* out /* 3 x n_bonds */);
 int rbmhost_perms(const rbmhost_model* m, int64_t* out /* n_symm x N */);
+/* cluster catalogue of the learned proposal (no counterpart in the
+ * reference): N x kmax, row i = the first lattice loop through site i
+ * (Kitaev hexagons, honeycomb.cpp:201-209; toric vertex stars,
+ * toric_lattice.cpp:57-69) starting at i */
+int rbmhost_clusters(const rbmhost_model* m, int64_t* out /* N x 8 */, int64_t* kmax);
 /* sites (k each, concatenated) and 2^k x 2^k matrices of the terms */
 int rbmhost_terms(const rbmhost_model* m, int64_t* k, int64_t* sites, double* mats, int64_t* groups);
 double rbmhost_exact_energy(const rbmhost_model* m);

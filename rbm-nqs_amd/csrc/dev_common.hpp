@@ -243,6 +243,15 @@ __device__ inline T group_sum(T v) {
         return v;
     }
 }
+template <int L, int O = 1>
+__device__ inline float group_max(float v) {
+    if constexpr (O < L) {
+        v = fmaxf(v, partner<O>(v));
+        return group_max<L, 2 * O>(v);
+    } else {
+        return v;
+    }
+}
 template <int L, int O = 1, typename R>
 __device__ inline cplx<R> group_csum(cplx<R> v) {
     if constexpr (O < L) {

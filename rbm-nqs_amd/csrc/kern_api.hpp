@@ -130,6 +130,63 @@ struct SxArgs {
     const CgStatus* status;
 };
 
+// learned proposal (no counterpart in the reference); see kern_learn.hpp
+struct LearnArgs {
+    SampArgs<float> s;
+    int Hn, Np, kmax;        // hidden width, N padded to 16, loop length
+    uint32_t lam_thr;        // learned move if the Philox word < lam_thr
+    const uint16_t* W1;      // bf16, Hn x Np (row h: weights of hidden unit h)
+    const uint16_t* W2;      // bf16, Np x Hn
+    const float* b1;         // Hn
+    const float* b2;         // Np
+    const uint8_t* cyc;      // N x kmax: cluster (i, k) = cyc[i][0..k-1]
+    const uint8_t* seedc;    // N x kmax: seeds generating the set of (i, k)
+    const uint8_t* seedl;    // N x kmax x 6
+    uint64_t* rec_st;        // per kept sample: state before the last learned proposal
+    int16_t* rec_seed;       // its seed (-1: none)
+    uint8_t* rec_k;          // its size
+    float* rec_a;            // its acceptance probability
+    uint32_t* lacc;          // per chain: accepted / proposed learned moves
+    uint32_t* lprop;
+};
+// LDS bytes of the proposal network region of k_sample_mlp (after the
+// sampler tables)
+__host__ __device__ inline size_t mlp_lds_bytes(int Hn, int Np, int N, int kmax) {
+    size_t b = 2 * (size_t)Hn * Np * 2 + ((size_t)Hn + Np) * 4;  // W1, W2 bf16, b1, b2
+    b += 16 * (size_t)Np * 2 + 16 * (size_t)Hn * 2 + 2 * 16 * (size_t)Np * 4;  // Sg, Ha, Zt, Zc
+    b += (size_t)N * kmax * 8;  // cyc, seedc, seedl
+    return (b + 15) & ~(size_t)15;
+}
+// chains (wavefronts) per k_sample_mlp block; the MFMA tiles have 16 rows,
+// rows kMlpChains..15 stay zero
+constexpr int kMlpChains = 8;
+hipError_t launch_sample_mlp(const LearnArgs& a, int K, hipStream_t st);
+hipError_t launch_mlp_logits(const LearnArgs& a, const uint64_t* states, int64_t n, float* z, hipStream_t st);
+
+// training of the proposal network on a batch of records (FP32)
+struct TrainArgs {
+    int N, Np, Hn, nw;
+    int64_t B;
+    const uint64_t* st;      // B x nw states
+    const int16_t* seed;     // B
+    const float* w;          // B rewards (already centred)
+    float beta, inv_b;       // entropy weight, 1 / B
+    const float* W1;         // fp32 masters, Hn x N (row-major)
+    const float* b1;
+    const float* W2;         // N x Hn
+    const float* b2;
+    float* S;                // scratch B x N (spins)
+    float* A;                // B x Hn (activations)
+    float* dZ;               // B x N
+    float* dU;               // B x Hn
+    float* loss;             // B
+    float* gW1;              // Hn x N
+    float* gb1;
+    float* gW2;              // N x Hn
+    float* gb2;
+};
+hipError_t launch_train_grad(const TrainArgs& a, hipStream_t st);
+
 // structured (basic RBM) S.x / moments, FP32 MFMA; see kern_struct.hpp
 struct StructArgs {
     int64_t S;
@@ -164,7 +221,7 @@ constexpr Shape kShapes[] = {{8, 1},  {8, 2},  {8, 4},  {8, 8},  {8, 16}, {16, 2
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <typename R>
-size_t lds_bytes(const Dims& d) {
+__host__ __device__ inline size_t lds_bytes(const Dims& d) {
     // [C | S | W? | rb | fs? | Pfaffian scratch?] complex R, [val?] complex
     // double, [rowptr? k? group? rowbase? | pb] int32, [bs?] int16,
     // [sites? | mask? | perm] bytes

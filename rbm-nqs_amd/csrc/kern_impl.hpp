@@ -178,6 +178,21 @@ struct Units {
             if (sizeof(R) == 4 && (u & 3) == 3) renorm(p, pe);
         }
     }
+    // accepted move of the flip set (fl[i], sg[i]) for mask bits i < 8
+    __device__ void accept_mask(const Dims& d, const Lds<R>& l, const int* fl, const R* sg, int mask) {
+        const cplx<R>* W = d.w_lds ? l.Wl : l.Wg;
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            if (a[u] < 0) continue;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (!((mask >> i) & 1)) continue;
+                cplx<R> w = W[kidx(d, l.P, s[u], fl[i]) * d.n_alpha + a[u]];
+                th[u] = th[u] - (2 * sg[i]) * w;
+            }
+            tn[u] = ctanh(th[u]);
+        }
+    }
     // accepted move: theta -= 2 sigma_f W (rbm_base.cpp:174-192)
     __device__ void accept12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1) {
         if (d.w_lds)

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -82,6 +83,29 @@ struct rbmgpu_ctx {
     std::vector<char> staging;
     hipEvent_t sxev[34] = {};
     hipEvent_t sxmid[16] = {};  // after the main S.x kernel of each matvec of a batch
+    // ---- learned proposal
+    int Hn = 0, Np = 0, kmax = 0;
+    double lam = 0;
+    int64_t learn_t = 0;  // Adam steps
+    std::vector<uint8_t> cyc, seedc, seedl;
+    uint8_t *d_cyc = nullptr, *d_seedc = nullptr, *d_seedl = nullptr;
+    uint16_t *d_W1b = nullptr, *d_W2b = nullptr;  // bf16 copies for the sampler
+    float *d_b1p = nullptr, *d_b2p = nullptr;      // biases (b2 padded to Np)
+    float* d_mlp = nullptr;    // masters [W1 | b1 | W2 | b2]
+    float* d_adam = nullptr;   // Adam moments m, v
+    float* d_grad = nullptr;   // gradient in the master layout
+    uint64_t* d_rec_st = nullptr;
+    int16_t* d_rec_seed = nullptr;
+    uint8_t* d_rec_k = nullptr;
+    float* d_rec_a = nullptr;
+    uint32_t *d_lacc = nullptr, *d_lprop = nullptr;
+    float* d_tr = nullptr;     // training scratch
+    int64_t tr_cap = 0;
+    uint64_t* d_tst = nullptr;
+    int16_t* d_tseed = nullptr;
+    float* d_tw = nullptr;
+    bool learned_epoch = false;  // the last epoch ran the learned sampler
+    int64_t mlp_n() const { return (int64_t)Hn * N + Hn + (int64_t)N * Hn + N; }
     std::string err;
 };
 
@@ -219,6 +243,12 @@ int ensure_samples(rbmgpu_ctx* c, int64_t S) {
         if (int rc = dalloc(ctx, &c->d_O, S * c->ld * 2 * rsize(c))) return rc;
     if (c->pf_rows)
         if (int rc = dalloc(ctx, &c->d_Opf, S * 4 * c->pf_rows * 2 * rsize(c))) return rc;
+    if (c->Hn) {
+        if (int rc = dalloc(ctx, &c->d_rec_st, S * c->nw * sizeof(uint64_t))) return rc;
+        if (int rc = dalloc(ctx, &c->d_rec_seed, S * sizeof(int16_t))) return rc;
+        if (int rc = dalloc(ctx, &c->d_rec_k, S)) return rc;
+        if (int rc = dalloc(ctx, &c->d_rec_a, S * sizeof(float))) return rc;
+    }
     c->cap = S;
     return 0;
 }
@@ -241,8 +271,7 @@ int pick_block(const rbmgpu_ctx* c, int L, int64_t groups) {
 }
 
 template <typename R>
-int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
-    rbmgpu_ctx* ctx = c;
+SampArgs<R> samp_args(rbmgpu_ctx* c, uint32_t epoch, int block, int64_t n_samples, int warmup) {
     SampArgs<R> a{};
     a.d = dims(c, block);
     a.t = tabs<R>(c);
@@ -265,7 +294,172 @@ int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samp
     a.pf_refresh = c->pf_refresh;
     a.acc = c->d_acc;
     a.prop = c->d_prop;
+    return a;
+}
+
+template <typename R>
+int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
+    rbmgpu_ctx* ctx = c;
+    SampArgs<R> a = samp_args<R>(c, epoch, block, n_samples, warmup);
     HIPCK(launch_sample<R>(a, s, block, c->stream));
+    return 0;
+}
+
+// ---- learned proposal
+LearnArgs learn_args(rbmgpu_ctx* c, uint32_t epoch) {
+    LearnArgs a{};
+    a.s = samp_args<float>(c, epoch, 64 * kMlpChains, c->n_samples, c->warmup);
+    a.Hn = c->Hn;
+    a.Np = c->Np;
+    a.kmax = c->kmax;
+    double thr = std::floor(c->lam * 4294967296.0);
+    a.lam_thr = (uint32_t)std::min(std::max(thr, 0.0), 4294967295.0);
+    a.W1 = c->d_W1b;
+    a.W2 = c->d_W2b;
+    a.b1 = c->d_b1p;
+    a.b2 = c->d_b2p;
+    a.cyc = c->d_cyc;
+    a.seedc = c->d_seedc;
+    a.seedl = c->d_seedl;
+    a.rec_st = c->d_rec_st;
+    a.rec_seed = c->d_rec_seed;
+    a.rec_k = c->d_rec_k;
+    a.rec_a = c->d_rec_a;
+    a.lacc = c->d_lacc;
+    a.lprop = c->d_lprop;
+    return a;
+}
+
+// smallest units-per-lane K of the learned kernel with 64 K >= H
+int learn_k(const rbmgpu_ctx* c) {
+    for (int k : {1, 2, 4, 5, 8})
+        if (64 * k >= c->H) return k;
+    return 0;
+}
+
+// bf16 copies and padded biases from the FP32 masters
+__global__ void k_mlp_pack(const float* mlp, int N, int Np, int Hn, uint16_t* W1b, uint16_t* W2b, float* b1p,
+                           float* b2p) {
+    const float* W1 = mlp;
+    const float* b1 = W1 + (int64_t)Hn * N;
+    const float* W2 = b1 + Hn;
+    const float* b2 = W2 + (int64_t)N * Hn;
+    auto bf = [](float f) {
+        uint32_t b = __float_as_uint(f);
+        b += 0x7FFFu + ((b >> 16) & 1u);
+        return (uint16_t)(b >> 16);
+    };
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)Hn * Np) {
+        int h = (int)(i / Np), j = (int)(i % Np);
+        W1b[i] = j < N ? bf(W1[(int64_t)h * N + j]) : 0;
+        int o = (int)(i / Hn), hh = (int)(i % Hn);
+        W2b[i] = o < N ? bf(W2[(int64_t)o * Hn + hh]) : 0;
+    }
+    if (i < Hn) b1p[i] = b1[i];
+    if (i < Np) b2p[i] = i < N ? b2[i] : 0.f;
+}
+
+// Adam (Kingma & Ba) on the FP32 masters
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float c1, float c2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1 - b1) * gi, vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+int learn_pack(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    int64_t n = std::max<int64_t>((int64_t)c->Hn * c->Np, std::max(c->Hn, c->Np));
+    hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_mlp, c->N,
+                       c->Np, c->Hn, c->d_W1b, c->d_W2b, c->d_b1p, c->d_b2p);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+int ensure_train(rbmgpu_ctx* c, int64_t B) {
+    rbmgpu_ctx* ctx = c;
+    if (B <= c->tr_cap) return 0;
+    // S (B x N), A (B x Hn), dZ (B x N), dU (B x Hn), loss (B), w (B)
+    if (int rc = dalloc(ctx, &c->d_tr, (size_t)B * (2 * c->N + 2 * c->Hn + 1) * sizeof(float))) return rc;
+    if (int rc = dalloc(ctx, &c->d_tst, (size_t)B * c->nw * sizeof(uint64_t))) return rc;
+    if (int rc = dalloc(ctx, &c->d_tseed, (size_t)B * sizeof(int16_t))) return rc;
+    if (int rc = dalloc(ctx, &c->d_tw, (size_t)B * sizeof(float))) return rc;
+    c->tr_cap = B;
+    return 0;
+}
+
+// gradient of the batch objective into d_grad; loss to *loss
+int learn_grad(rbmgpu_ctx* c, int64_t B, double beta, double* loss) {
+    rbmgpu_ctx* ctx = c;
+    TrainArgs t{};
+    t.N = c->N;
+    t.Np = c->Np;
+    t.Hn = c->Hn;
+    t.nw = c->nw;
+    t.B = B;
+    t.st = c->d_tst;
+    t.seed = c->d_tseed;
+    t.w = c->d_tw;
+    t.beta = (float)beta;
+    t.inv_b = 1.f / (float)B;
+    t.W1 = c->d_mlp;
+    t.b1 = t.W1 + (int64_t)c->Hn * c->N;
+    t.W2 = t.b1 + c->Hn;
+    t.b2 = t.W2 + (int64_t)c->N * c->Hn;
+    t.S = c->d_tr;
+    t.A = t.S + B * c->N;
+    t.dZ = t.A + B * c->Hn;
+    t.dU = t.dZ + B * c->N;
+    t.loss = t.dU + B * c->Hn;
+    t.gW1 = c->d_grad;
+    t.gb1 = t.gW1 + (int64_t)c->Hn * c->N;
+    t.gW2 = t.gb1 + c->Hn;
+    t.gb2 = t.gW2 + (int64_t)c->N * c->Hn;
+    HIPCK(launch_train_grad(t, c->stream));
+    std::vector<float> lv(B);
+    HIPCK(hipMemcpyAsync(lv.data(), t.loss, B * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    double L = 0;
+    for (float x : lv) L += x;
+    if (loss) *loss = L / (double)B;
+    return 0;
+}
+
+// cluster catalogue tables: cyc (N x kmax) and, per (i, k), the seeds j
+// whose first-k clusters are the same site set
+int learn_tables(rbmgpu_ctx* c, const int32_t* clusters) {
+    const int N = c->N, K = c->kmax;
+    c->cyc.assign((size_t)N * K, 0);
+    c->seedc.assign((size_t)N * K, 0);
+    c->seedl.assign((size_t)N * K * 6, 0);
+    for (int i = 0; i < N; i++)
+        for (int t = 0; t < K; t++) {
+            int v = clusters[i * K + t];
+            if (v < 0 || v >= N) return fail(c, "cluster site out of range");
+            c->cyc[i * K + t] = (uint8_t)v;
+        }
+    for (int i = 0; i < N; i++)
+        if (c->cyc[i * K] != i) return fail(c, "cluster of site i must start at i");
+    for (int i = 0; i < N; i++)
+        for (int k = 1; k <= K; k++) {
+            std::vector<int> a(c->cyc.begin() + i * K, c->cyc.begin() + i * K + k);
+            std::sort(a.begin(), a.end());
+            int cnt = 0;
+            for (int j = 0; j < N; j++) {
+                std::vector<int> b(c->cyc.begin() + j * K, c->cyc.begin() + j * K + k);
+                std::sort(b.begin(), b.end());
+                if (a == b) {
+                    if (cnt == 6) return fail(c, "more than 6 seeds of one cluster");
+                    c->seedl[(i * K + k - 1) * 6 + cnt++] = (uint8_t)j;
+                }
+            }
+            c->seedc[i * K + k - 1] = (uint8_t)cnt;
+        }
     return 0;
 }
 
@@ -958,6 +1152,38 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     }
     c->params.assign(c->P, 0.);
     if (build_weight_tables(c)) return bad(c->err);
+    if (cfg->learn_hidden > 0) {  // learned proposal
+        if (c->prec != RBMGPU_FP32 || c->pf_rows) return bad("the learned proposal needs FP32 and no Pfaffian factor");
+        if (cfg->learn_hidden % 16 || cfg->learn_hidden > 512) return bad("learn_hidden: multiple of 16, <= 512");
+        if (cfg->cluster_kmax < 1 || cfg->cluster_kmax > 8 || !cfg->clusters) return bad("invalid cluster table");
+        c->Hn = cfg->learn_hidden;
+        c->Np = (c->N + 15) / 16 * 16;
+        c->kmax = cfg->cluster_kmax;
+        c->lam = cfg->learn_fraction;
+        if (!learn_k(c)) return bad("too many hidden units for the learned sampler");
+        if (learn_tables(c, cfg->clusters)) return bad(c->err);
+        if (upload(ctx, &c->d_cyc, c->cyc.data(), c->cyc.size())) return bad(c->err);
+        if (upload(ctx, &c->d_seedc, c->seedc.data(), c->seedc.size())) return bad(c->err);
+        if (upload(ctx, &c->d_seedl, c->seedl.data(), c->seedl.size())) return bad(c->err);
+        // initial weights: W1 ~ N(0, 1/N), W2 ~ N(0, 1e-4) (near-uniform seeds), zero biases
+        std::mt19937_64 rng(cfg->learn_seed);
+        std::normal_distribution<float> d1(0.f, 1.f / std::sqrt((float)c->N)), d2(0.f, 1e-2f);
+        std::vector<float> mlp(c->mlp_n(), 0.f);
+        for (int64_t i = 0; i < (int64_t)c->Hn * c->N; i++) mlp[i] = d1(rng);
+        for (int64_t i = 0; i < (int64_t)c->N * c->Hn; i++) mlp[(int64_t)c->Hn * c->N + c->Hn + i] = d2(rng);
+        if (upload(ctx, &c->d_mlp, mlp.data(), mlp.size() * 4)) return bad(c->err);
+        std::vector<float> z(2 * mlp.size(), 0.f);
+        if (upload(ctx, &c->d_adam, z.data(), z.size() * 4)) return bad(c->err);
+        if (dalloc(ctx, &c->d_grad, mlp.size() * 4)) return bad(c->err);
+        if (dalloc(ctx, &c->d_W1b, (size_t)c->Hn * c->Np * 2) || dalloc(ctx, &c->d_W2b, (size_t)c->Hn * c->Np * 2) ||
+            dalloc(ctx, &c->d_b1p, (size_t)c->Hn * 4) || dalloc(ctx, &c->d_b2p, (size_t)c->Np * 4))
+            return bad(c->err);
+        if (dalloc(ctx, &c->d_lacc, c->n_chains * 4) || dalloc(ctx, &c->d_lprop, c->n_chains * 4)) return bad(c->err);
+        if (learn_pack(c)) return bad(c->err);
+        size_t lds = lds_bytes<float>(dims(c, 64 * kMlpChains)) + mlp_lds_bytes(c->Hn, c->Np, c->N, c->kmax);
+        if (lds > c->lds_limit) return bad("the learned sampler's tables do not fit LDS");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bad("stream synchronisation failed");
+    }
     *out = c;
     return 0;
 }
@@ -970,7 +1196,11 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_tsites, c->d_mask,   c->d_val,    c->d_states, c->d_obs,   c->d_T,      c->d_O,
                     c->d_acc,    c->d_prop,   c->d_part,   c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
                     c->d_emom,   c->d_x,      c->d_b,      c->d_r,     c->d_p,      c->d_Ap,     c->d_st,
-                    c->d_sxpart, c->d_sxpart2, c->d_evst,   c->d_evout,  c->d_evobs};
+                    c->d_sxpart, c->d_sxpart2, c->d_evst,   c->d_evout,  c->d_evobs, c->d_bs,
+                    c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,
+                    c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,
+                    c->d_rec_k,  c->d_rec_a,  c->d_lacc,   c->d_lprop, c->d_tr,     c->d_tst,    c->d_tseed,
+                    c->d_tw};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev)
@@ -1027,10 +1257,16 @@ int rbmgpu_sample_epoch(rbmgpu_ctx* c, uint32_t epoch) {
     if (int rc = ensure_samples(c, c->n_samples)) return rc;
     HIPCK(hipMemsetAsync(c->d_obs, 0, c->n_samples * 2 * c->n_groups * sizeof(double), c->stream));
     HIPCK(hipEventRecord(c->ev[0], c->stream));
-    int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, epoch, c->shape, c->block, c->n_samples, c->warmup)
+    int rc = 0;
+    if (c->Hn && c->lam > 0) {
+        HIPCK(launch_sample_mlp(learn_args(c, epoch), learn_k(c), c->stream));
+    } else {
+        rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, epoch, c->shape, c->block, c->n_samples, c->warmup)
                                     : run_sample<double>(c, epoch, c->shape, c->block, c->n_samples, c->warmup);
+    }
     if (rc) return rc;
     HIPCK(hipEventRecord(c->ev[1], c->stream));
+    c->learned_epoch = c->Hn && c->lam > 0;
     c->S = c->n_samples;
     c->have_chains = true;
     rc = c->prec == RBMGPU_FP32 ? run_build_O<float>(c) : run_build_O<double>(c);
@@ -1112,6 +1348,140 @@ int rbmgpu_apply_S(rbmgpu_ctx* c, const double* x, double* y, double r1, double 
     HIPCK(hipMemcpyAsync(y, c->d_Ap, c->P * 16, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     c->times[3] = elapsed(c, 6, 7);
+    return 0;
+}
+
+int rbmgpu_learn_set_fraction(rbmgpu_ctx* c, double fraction) {
+    if (!c || fraction < 0 || fraction > 1) return -1;
+    if (!c->Hn && fraction > 0) return fail(c, "no proposal network (learn_hidden = 0)");
+    c->lam = fraction;
+    return 0;
+}
+
+int rbmgpu_learn_weights(rbmgpu_ctx* c, int32_t set, double* params) {
+    if (!c || !params) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->Hn) return fail(c, "no proposal network (learn_hidden = 0)");
+    std::vector<float> f(c->mlp_n());
+    if (set) {
+        for (size_t i = 0; i < f.size(); i++) f[i] = (float)params[i];
+        HIPCK(hipMemcpyAsync(c->d_mlp, f.data(), f.size() * 4, hipMemcpyHostToDevice, c->stream));
+        if (int rc = learn_pack(c)) return rc;
+        HIPCK(hipStreamSynchronize(c->stream));
+    } else {
+        HIPCK(hipMemcpyAsync(f.data(), c->d_mlp, f.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < f.size(); i++) params[i] = f[i];
+    }
+    return 0;
+}
+
+int rbmgpu_learn_logits(rbmgpu_ctx* c, const uint64_t* states, int64_t n, double* z) {
+    if (!c || !states || !z || n < 1) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->Hn) return fail(c, "no proposal network (learn_hidden = 0)");
+    if (int rc = ensure_eval(c, n)) return rc;
+    float* dz = nullptr;
+    HIPCK(hipMalloc((void**)&dz, (size_t)n * c->N * 4));
+    HIPCK(hipMemcpyAsync(c->d_evst, states, n * c->nw * 8, hipMemcpyHostToDevice, c->stream));
+    LearnArgs a = learn_args(c, 0);
+    hipError_t e = launch_mlp_logits(a, c->d_evst, n, dz, c->stream);
+    std::vector<float> h((size_t)n * c->N);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dz, h.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dz);
+    HIPCK(e);
+    for (size_t i = 0; i < h.size(); i++) z[i] = h[i];
+    return 0;
+}
+
+int rbmgpu_learn_grad(rbmgpu_ctx* c, const uint64_t* states, const int32_t* seeds, const double* w, int64_t n,
+                      double beta, double* grads, double* loss) {
+    if (!c || !states || !seeds || !w || !grads || n < 1) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->Hn) return fail(c, "no proposal network (learn_hidden = 0)");
+    if (int rc = ensure_train(c, n)) return rc;
+    std::vector<int16_t> sd(n);
+    std::vector<float> wf(n);
+    for (int64_t i = 0; i < n; i++) {
+        if (seeds[i] < -1 || seeds[i] >= c->N) return fail(c, "seed out of range");
+        sd[i] = (int16_t)seeds[i];
+        wf[i] = (float)w[i];
+    }
+    HIPCK(hipMemcpyAsync(c->d_tst, states, n * c->nw * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->d_tseed, sd.data(), n * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->d_tw, wf.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    if (int rc = learn_grad(c, n, beta, loss)) return rc;
+    std::vector<float> g(c->mlp_n());
+    HIPCK(hipMemcpyAsync(g.data(), c->d_grad, g.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < g.size(); i++) grads[i] = g[i];
+    return 0;
+}
+
+int rbmgpu_learn_train(rbmgpu_ctx* c, double lr, double beta, int64_t batch, double* out4) {
+    if (!c || batch < 1) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->Hn) return fail(c, "no proposal network (learn_hidden = 0)");
+    if (!c->learned_epoch || c->S == 0) return fail(c, "no records: sample an epoch with the learned sampler first");
+    const int64_t S = c->S;
+    std::vector<uint64_t> st(S * c->nw);
+    std::vector<int16_t> sd(S);
+    std::vector<uint8_t> kk(S);
+    std::vector<float> aa(S);
+    std::vector<uint32_t> lacc(c->n_chains), lprop(c->n_chains);
+    HIPCK(hipMemcpyAsync(st.data(), c->d_rec_st, st.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(sd.data(), c->d_rec_seed, S * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(kk.data(), c->d_rec_k, S, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(aa.data(), c->d_rec_a, S * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(lacc.data(), c->d_lacc, c->n_chains * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(lprop.data(), c->d_lprop, c->n_chains * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    // batch: valid records spread evenly over the sample set
+    std::vector<int64_t> idx;
+    for (int64_t r = 0; r < S; r++)
+        if (sd[r] >= 0) idx.push_back(r);
+    double np = 0, na = 0;
+    for (int i = 0; i < c->n_chains; i++) np += lprop[i], na += lacc[i];
+    if (out4) {
+        out4[0] = 0;
+        out4[1] = 0;
+        out4[2] = np;
+        out4[3] = na;
+    }
+    if (idx.empty()) return 0;
+    int64_t B = std::min<int64_t>(batch, (int64_t)idx.size());
+    std::vector<uint64_t> bst(B * c->nw);
+    std::vector<int16_t> bsd(B);
+    std::vector<float> bw(B);
+    double mean = 0;
+    for (int64_t b = 0; b < B; b++) {
+        int64_t r = idx[(size_t)((double)b * idx.size() / B)];
+        for (int q = 0; q < c->nw; q++) bst[b * c->nw + q] = st[r * c->nw + q];
+        bsd[b] = sd[r];
+        bw[b] = aa[r] * kk[r];  // reward: expected number of spins moved
+        mean += bw[b];
+    }
+    mean /= B;
+    for (auto& x : bw) x = (float)(x - mean);
+    if (int rc = ensure_train(c, B)) return rc;
+    HIPCK(hipMemcpyAsync(c->d_tst, bst.data(), bst.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->d_tseed, bsd.data(), B * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->d_tw, bw.data(), B * 4, hipMemcpyHostToDevice, c->stream));
+    double loss = 0;
+    if (int rc = learn_grad(c, B, beta, &loss)) return rc;
+    c->learn_t++;
+    const int64_t n = c->mlp_n();
+    float c1 = 1.f - std::pow(0.9f, (float)c->learn_t), c2 = 1.f - std::pow(0.999f, (float)c->learn_t);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_mlp, c->d_grad,
+                       c->d_adam, c->d_adam + n, n, (float)lr, c1, c2);
+    HIPCK(hipGetLastError());
+    if (int rc = learn_pack(c)) return rc;
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (out4) {
+        out4[0] = loss;
+        out4[1] = mean;
+    }
     return 0;
 }
 

@@ -51,6 +51,8 @@ int rbmhost_info(const rbmhost_model* m, int64_t* o) {
     o[12] = (int64_t)s.warmup();
     o[13] = (int64_t)cfg.sa_metropolis_n_steps_per_sample;
     o[14] = (int64_t)s.prop_bonds().size() / 2;
+    o[15] = cfg.learn_hidden;
+    o[16] = s.cluster_kmax();
     return 0;
 }
 
@@ -69,6 +71,14 @@ int rbmhost_perms(const rbmhost_model* m, int64_t* out) {
     if (!m || !out) return -1;
     const auto& p = m->s->layout().perm;
     for (size_t i = 0; i < p.size(); i++) out[i] = p[i];
+    return 0;
+}
+
+int rbmhost_clusters(const rbmhost_model* m, int64_t* out, int64_t* kmax) {
+    if (!m || !out || !kmax) return -1;
+    const auto& c = m->s->clusters();
+    for (size_t i = 0; i < c.size(); i++) out[i] = c[i];
+    *kmax = m->s->cluster_kmax();
     return 0;
 }
 

@@ -74,6 +74,12 @@ struct config {
     int gpu_lanes = 0;
     // materialise the O matrix even where the structured S.x applies
     bool gpu_keep_o = false;
+    // learned cluster proposal (new keys; the reference has none)
+    int learn_hidden = 0;          // hidden width of the proposal network, 0: off
+    double learn_fraction = 0.5;   // probability of a learned move per slot
+    double learn_lr = 1e-3;        // Adam learning rate
+    double learn_entropy = 0.01;   // entropy weight of the objective
+    int64_t learn_batch = 16384;   // records per training step
 };
 
 // Parse .ini text into cfg (unknown keys are an error, like

@@ -99,6 +99,28 @@ setup::setup(const ini::config& cfg) : cfg_(cfg) {
             prop_bonds_.push_back((int32_t)b.b);
         }
     for (const auto& op : model_->get_hamiltonian()) terms_.add(op, 0);
+    {  // cluster catalogue (learned proposal)
+        std::vector<std::vector<size_t>> loops;
+        if (cfg.model == "kitaev") {
+            loops = static_cast<model::kitaev*>(model_.get())->honeycomb_lattice().get_hexagons();
+        } else {
+            for (const auto& p : static_cast<lattice::toric_lattice&>(lat).construct_plaqs())
+                if (p.type == 1) loops.push_back({p.idxs.begin(), p.idxs.end()});
+        }
+        cluster_kmax_ = loops.empty() ? 0 : (int)loops[0].size();
+        for (int i = 0; i < layout_.n_spins; i++) {
+            bool found = false;
+            for (const auto& l : loops) {
+                auto it = std::find(l.begin(), l.end(), (size_t)i);
+                if (it == l.end()) continue;
+                size_t p0 = (size_t)(it - l.begin());
+                for (int t = 0; t < cluster_kmax_; t++) clusters_.push_back((int32_t)l[(p0 + t) % l.size()]);
+                found = true;
+                break;
+            }
+            if (!found) throw std::runtime_error("site on no lattice loop");
+        }
+    }
     n_samples_ = cfg.sa_n_samples;
     warmup_ = cfg.sa_metropolis_n_warmup_steps;
 }
@@ -149,6 +171,11 @@ rbmgpu_config setup::gpu_config() const {
     c.precision = cfg_.gpu_precision;
     c.lanes_per_chain = cfg_.gpu_lanes;
     c.keep_o = cfg_.gpu_keep_o ? 1 : 0;
+    c.learn_hidden = cfg_.learn_hidden;
+    c.learn_fraction = cfg_.learn_fraction;
+    c.cluster_kmax = cluster_kmax_;
+    c.clusters = clusters_.data();
+    c.learn_seed = cfg_.seed;
     return c;
 }
 

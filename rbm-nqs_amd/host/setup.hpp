@@ -36,6 +36,11 @@ class setup {
     const machine_layout& layout() const { return layout_; }
     const term_table& terms() const { return terms_; }
     const std::vector<int32_t>& prop_bonds() const { return prop_bonds_; }
+    // cluster catalogue of the learned proposal: N x kmax, row i = the
+    // first lattice loop (Kitaev hexagon, toric vertex star) through site i,
+    // starting at i
+    const std::vector<int32_t>& clusters() const { return clusters_; }
+    int cluster_kmax() const { return cluster_kmax_; }
 
     // Hamiltonian only (training), or Hamiltonian without helper terms plus
     // one group per hexagon plaquette (--evaluate on Kitaev, main.cpp:640-680)
@@ -56,6 +61,8 @@ class setup {
     machine_layout layout_;
     term_table terms_;
     std::vector<int32_t> prop_bonds_;
+    std::vector<int32_t> clusters_;
+    int cluster_kmax_ = 0;
     size_t n_samples_, warmup_;
 };
 

@@ -56,13 +56,16 @@ def exported_symbols():
     return {
         "gpu": ["rbmgpu_create", "rbmgpu_destroy", "rbmgpu_last_error", "rbmgpu_n_params",
                 "rbmgpu_launch_shape", "rbmgpu_set_params", "rbmgpu_get_params", "rbmgpu_time_sx",
+                "rbmgpu_learn_train", "rbmgpu_learn_grad", "rbmgpu_learn_weights", "rbmgpu_learn_logits",
+                "rbmgpu_learn_set_fraction",
                 "rbmgpu_update_weights", "rbmgpu_sample_epoch", "rbmgpu_load_samples",
                 "rbmgpu_reduce", "rbmgpu_sr_solve", "rbmgpu_apply_S", "rbmgpu_eval_logpsi",
                 "rbmgpu_psi_ratio", "rbmgpu_get_samples", "rbmgpu_n_samples",
                 "rbmgpu_kernel_times", "rbmgpu_synchronize"],
         "host": ["rbmhost_create", "rbmhost_destroy", "rbmhost_info", "rbmhost_bonds",
                  "rbmhost_perms", "rbmhost_terms", "rbmhost_exact_energy", "rbmhost_use_evaluation",
-                 "rbmhost_set_n_samples", "rbmhost_initial_weights", "rbmhost_gpu_create"],
+                 "rbmhost_set_n_samples", "rbmhost_initial_weights", "rbmhost_gpu_create",
+                 "rbmhost_clusters"],
     }
 
 
@@ -83,11 +86,11 @@ class Model:
 
     def _refresh(self):
         _, h = libs()
-        o = np.zeros(15, dtype=np.int64)
+        o = np.zeros(17, dtype=np.int64)
         h.rbmhost_info(C.c_void_p(self._h), _p(o, I64P))
         (self.N, self.P, self.n_neural, self.n_vb, self.n_alpha, self.n_symm, self.pf_rows,
          self.n_terms, self.n_groups, self.n_bonds, self.n_chains, self.n_samples, self.n_warmup,
-         self.step_size, self.n_prop_bonds) = [int(x) for x in o]
+         self.step_size, self.n_prop_bonds, self.learn_hidden, self.cluster_kmax) = [int(x) for x in o]
         self.n_words = (self.N + 63) // 64
         self.H = self.n_alpha * self.n_symm
 
@@ -109,6 +112,13 @@ class Model:
         out = np.zeros(self.n_symm * self.N, dtype=np.int64)
         libs()[1].rbmhost_perms(self.h, _p(out, I64P))
         return out.reshape(self.n_symm, self.N)
+
+    def clusters(self):
+        """cluster catalogue of the learned proposal (N x kmax)"""
+        out = np.zeros(8 * self.N, dtype=np.int64)
+        k = C.c_int64()
+        libs()[1].rbmhost_clusters(self.h, _p(out, I64P), C.byref(k))
+        return out[: self.N * k.value].reshape(self.N, k.value)
 
     def terms(self):
         k = np.zeros(self.n_terms, dtype=np.int64)
@@ -168,6 +178,7 @@ class Gpu:
         self.n_words = model.n_words
         self.n_groups = model.n_groups
         self.n_chains = model.n_chains
+        self.Hn = model.learn_hidden
 
     def __del__(self):
         if getattr(self, "_c", None):
@@ -282,6 +293,44 @@ class Gpu:
         t = np.zeros(6)
         self._chk(libs()[0].rbmgpu_kernel_times(self.c, _p(t, DP)))
         return dict(zip(["sample", "assemble", "reduce", "sx", "cg", "sx_main"], t.tolist()))
+
+    # ---- learned proposal
+    def learn_weights(self, params=None):
+        """get (params None) or set the FP32 master weights [W1|b1|W2|b2]"""
+        n = self.Hn * self.N * 2 + self.Hn + self.N
+        if params is None:
+            out = np.zeros(n)
+            self._chk(libs()[0].rbmgpu_learn_weights(self.c, C.c_int32(0), _p(out, DP)))
+            return out
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        assert p.size == n
+        self._chk(libs()[0].rbmgpu_learn_weights(self.c, C.c_int32(1), _p(p, DP)))
+
+    def learn_grad(self, states, seeds, w, beta):
+        st = np.ascontiguousarray(states, dtype=np.uint64)
+        sd = np.ascontiguousarray(seeds, dtype=np.int32)
+        ww = np.ascontiguousarray(w, dtype=np.float64)
+        g = np.zeros(self.Hn * self.N * 2 + self.Hn + self.N)
+        loss = C.c_double()
+        self._chk(libs()[0].rbmgpu_learn_grad(self.c, _p(st, U64P), _p(sd, I32P), _p(ww, DP),
+                                              C.c_int64(len(sd)), C.c_double(beta), _p(g, DP), C.byref(loss)))
+        return loss.value, g
+
+    def learn_logits(self, states):
+        st = np.ascontiguousarray(states, dtype=np.uint64)
+        n = st.shape[0]
+        z = np.zeros(n * self.N)
+        self._chk(libs()[0].rbmgpu_learn_logits(self.c, _p(st, U64P), C.c_int64(n), _p(z, DP)))
+        return z.reshape(n, self.N)
+
+    def learn_train(self, lr=1e-3, beta=0.01, batch=16384):
+        out = np.zeros(4)
+        self._chk(libs()[0].rbmgpu_learn_train(self.c, C.c_double(lr), C.c_double(beta), C.c_int64(batch),
+                                               _p(out, DP)))
+        return dict(loss=out[0], reward=out[1], proposed=out[2], accepted=out[3])
+
+    def learn_set_fraction(self, f):
+        self._chk(libs()[0].rbmgpu_learn_set_fraction(self.c, C.c_double(f)))
 
     def time_sx(self, reps=20):
         """mean ms of the main S.x kernel over reps back-to-back launches"""

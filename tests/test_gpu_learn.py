@@ -1,0 +1,135 @@
+"""GPU tests of the learned cluster proposal (no counterpart in the
+reference; its pins are the FP64 oracle network and gradient, which are
+checked against finite differences in test_oracle_learn.py, and exact
+enumeration for the sampled distribution).
+
+Tolerances: the training forward/backward runs in FP32 -> gradients within
+1e-4 of the largest gradient entry; the sampler's forward runs bf16 MFMA
+with bf16 hidden activations -> logits within 2e-2 of the logit scale
+against the oracle evaluated on the same bf16-rounded weights.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import rbmgpu
+from helpers import ini, random_params
+
+pytestmark = pytest.mark.gpu
+
+
+def learn_spec(model, n, hidden, fraction, chains, samples, alpha=1, rbm="basic", warmup=100):
+    return ini(model, n, rbm=rbm, alpha=alpha, extra=f"""
+[sampler]
+n_samples = {samples}
+metropolis.n_chains = {chains}
+metropolis.n_warmup_steps = {warmup}
+[learn]
+hidden = {hidden}
+fraction = {fraction}
+""")
+
+
+def _net(rng, N, Hn, s1=0.4, s2=0.3):
+    W1 = s1 * rng.normal(size=(Hn, N))
+    b1 = 0.1 * rng.normal(size=Hn)
+    W2 = s2 * rng.normal(size=(N, Hn))
+    b2 = 0.1 * rng.normal(size=N)
+    return W1, b1, W2, b2
+
+
+def _flat(W1, b1, W2, b2):
+    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2])
+
+
+def _bf16(x):
+    """round to bfloat16 (nearest even), returned as float64"""
+    f = np.asarray(x, dtype=np.float32).copy()
+    b = f.view(np.uint32)
+    b += np.uint32(0x7FFF) + ((b >> np.uint32(16)) & np.uint32(1))
+    b &= np.uint32(0xFFFF0000)
+    return b.view(np.float32).astype(np.float64)
+
+
+def _gpu(spec, seed=0, std=0.2):
+    print(f"[gpu-test learn] {spec.split()[3:6]}", flush=True)
+    m = rbmgpu.Model(spec)
+    g = m.gpu(precision=32)
+    rng = np.random.default_rng(seed)
+    g.set_params(random_params(rng, m.P, std))
+    return m, g, rng
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.05])
+def test_learn_grad_matches_oracle(beta):
+    m, g, rng = _gpu(learn_spec("kitaev", 3, 48, 0.5, 4, 64))
+    N, Hn = m.N, 48
+    net = _net(rng, N, Hn)
+    g.learn_weights(_flat(*net))
+    net32 = [np.asarray(x, dtype=np.float32).astype(np.float64) for x in net]
+    n = 40
+    st = po.random_states(rng, n, N)
+    seeds = rng.integers(0, N, n)
+    seeds[3] = -1  # a record without a learned proposal contributes nothing
+    w = rng.normal(size=n)
+    loss, gg = g.learn_grad(st, seeds, w, beta)
+    keep = seeds >= 0
+    # the GPU averages over all n records; the oracle over the ones it is given
+    lo, go = po.mlp_grad(*net32, st[keep], N, seeds[keep], w[keep], beta)
+    scale = keep.sum() / n
+    assert abs(loss - lo * scale) <= 1e-4 * (1 + abs(lo))
+    ref = np.concatenate([x.ravel() for x in go]) * scale
+    assert np.max(np.abs(gg - ref)) <= 1e-4 * np.abs(ref).max(), np.max(np.abs(gg - ref)) / np.abs(ref).max()
+
+
+def test_learn_logits_bf16_mfma_matches_oracle():
+    """the sampler's bf16 MFMA forward (operand layouts of
+    v_mfma_f32_16x16x16_bf16) against the FP64 network on bf16 weights"""
+    m, g, rng = _gpu(learn_spec("kitaev", 3, 64, 0.5, 4, 64))
+    N, Hn = m.N, 64
+    net = _net(rng, N, Hn, 0.3, 0.2)
+    g.learn_weights(_flat(*net))
+    st = po.random_states(rng, 21, N)
+    z = g.learn_logits(st)
+    W1, b1, W2, b2 = net
+    zo = po.mlp_logits(_bf16(W1), np.float32(b1), _bf16(W2), np.float32(b2), st, N)
+    assert np.max(np.abs(z - zo)) <= 2e-2 * (1 + np.abs(zo).max()), np.max(np.abs(z - zo))
+
+
+@pytest.mark.parametrize("fraction", [0.6, 1.0])
+def test_learned_sampler_matches_exact_distribution(fraction):
+    """Metropolis-Hastings with the learned cluster proposal (untrained,
+    non-uniform network) samples |psi|^2 exactly: <E> against exact
+    enumeration within 5 binned standard errors"""
+    spec = learn_spec("kitaev", 2, 32, fraction, 256, 32768, warmup=50)
+    m, g, rng = _gpu(spec, seed=5, std=0.4)
+    o = po.Oracle(spec)
+    p = random_params(np.random.default_rng(5), m.P, 0.4)
+    o.set_params(p)
+    g.learn_weights(_flat(*_net(rng, m.N, 32, 0.8, 0.8)))
+    exact = o.exact_expectation()
+    means, stds = [], []
+    for ep in range(4):
+        g.sample_epoch(epoch=ep)
+        r = g.reduce(want_o=False)
+        means.append(r["mean"].real)
+        stds.append(r["std"])
+    est = np.mean(means)
+    err = np.sqrt(np.mean(np.square(stds)) / len(stds))
+    assert abs(est - exact.real) < 5 * err + 1e-4, (est, exact, err)
+
+
+def test_learned_sampler_trains():
+    """sampling -> records -> REINFORCE/Adam steps: finite loss, weights
+    move, learned moves are proposed and accepted"""
+    spec = learn_spec("kitaev", 3, 64, 0.5, 256, 8192, warmup=20)
+    m, g, rng = _gpu(spec, seed=7, std=0.1)
+    w0 = g.learn_weights()
+    stats = []
+    for ep in range(3):
+        g.sample_epoch(epoch=ep)
+        stats.append(g.learn_train(lr=1e-2, beta=0.01, batch=4096))
+    w1 = g.learn_weights()
+    assert all(np.isfinite(s["loss"]) for s in stats)
+    assert stats[-1]["proposed"] > 0 and stats[-1]["accepted"] > 0
+    assert np.max(np.abs(w1 - w0)) > 1e-4

@@ -88,3 +88,10 @@ def test_eval_samples_rescale_warmup():
     m = rbmgpu.Model(ini("toric", 2, extra="[sampler]\nn_samples = 1000\nmetropolis.n_warmup_steps = 100\n"))
     m.set_n_samples(5000)
     assert m.n_samples == 5000 and m.n_warmup == 500
+
+
+@pytest.mark.parametrize("model,n", [("kitaev", 2), ("kitaev", 3), ("toric", 3), ("toric", 4)])
+def test_cluster_catalogue_matches_oracle(model, n):
+    """learned-proposal clusters: the first lattice loop through each site"""
+    spec = ini(model, n)
+    assert np.array_equal(rbmgpu.Model(spec).clusters(), po.Oracle(spec).clusters())
