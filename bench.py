@@ -106,7 +106,7 @@ def cpu_baseline():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    chains, spc = 64, 32
+    chains, spc = 512, 32
     o = po.Oracle(spec(chains, spc))
     rng = np.random.default_rng(421390484)
     o.set_params(WORKLOAD["weights_std"] * (rng.normal(size=o.P) + 1j * rng.normal(size=o.P)))
@@ -154,6 +154,69 @@ def sx_roofline(g, m, S, sx_ms):
             "traffic": None, "launch_ms": sx_ms}
 
 
+def c3_spec(learn, chains=16384, spc=8, warmup=100, hidden=256, fraction=0.5):
+    s = f"""
+[model]
+type = kitaev
+n_cells = 6
+J = -1
+symmetry = 0.5
+[rbm]
+type = symmetry
+alpha = 4
+weights = 0.05
+[sampler]
+n_samples = {chains * spc}
+metropolis.n_chains = {chains}
+metropolis.n_warmup_steps = {warmup}
+"""
+    if learn:
+        s += f"[learn]\nhidden = {hidden}\nfraction = {fraction}\n"
+    return s
+
+
+def plain_vs_learned(epochs=3, train_epochs=5, target=1e-4):
+    """Kitaev 6x6 symmetric RBM alpha=4: plain Metropolis (single and bond
+    flips) against the learned cluster proposal (72 -> 256 -> 72 network,
+    bf16 MFMA, trained for train_epochs REINFORCE/Adam steps on its own
+    records first).  Per epoch: sampler time, binned tau and std of E/N;
+    time to a fixed error of E/N = epoch time x (std / target)^2."""
+    import rbmgpu
+
+    out = {"workload": "Kitaev honeycomb 6x6 (N=72), symmetric RBM alpha=4 (P=293), weights N(0,0.05^2), "
+                       "16384 chains x 8 samples, 100 warm-up sweeps, FP32 sampler",
+           "target_error_per_site": target}
+    for learn in (False, True):
+        m = rbmgpu.Model(c3_spec(learn))
+        g = m.gpu(precision=32)
+        g.set_params(m.initial_weights(421390484))
+        tr = []
+        if learn:
+            for e in range(train_epochs):
+                g.sample_epoch(epoch=1000 + e)
+                tr.append(g.learn_train(lr=3e-3, beta=0.01, batch=16384))
+        ms, taus, stds, accs, es = [], [], [], [], []
+        for e in range(epochs):
+            g.sample_epoch(epoch=e)
+            r = g.reduce(want_o=False)
+            ms.append(g.kernel_times()["sample"])
+            taus.append(r["tau"])
+            stds.append(r["std"] / m.N)
+            accs.append(r["acc"])
+            es.append(r["mean"].real / m.N)
+        t = float(np.mean(ms))
+        sd = float(np.sqrt(np.mean(np.square(stds))))
+        rec = {"sampler_ms_per_epoch": t, "samples_per_s": m.n_samples / (t * 1e-3),
+               "tau_int": float(np.mean(taus)), "acceptance": float(np.mean(accs)),
+               "energy_per_site": float(np.mean(es)), "energy_stddev_per_site": sd,
+               "ms_to_target_error": t * (sd / target) ** 2}
+        if learn:
+            rec.update({"hidden": 256, "learned_fraction": 0.5, "training_epochs": train_epochs,
+                        "learned_acceptance": tr[-1]["accepted"] / max(tr[-1]["proposed"], 1)})
+        out["learned" if learn else "plain"] = rec
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters (run under rocprofv3 --pmc)."""
@@ -174,6 +237,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the plain vs learned proposal comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -225,6 +289,8 @@ def main():
         "launch_shape": list(g.launch_shape()),
         "roofline": roofline,
     }
+    if not a.no_learned:
+        out["plain_vs_learned"] = plain_vs_learned()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
