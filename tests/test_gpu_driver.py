@@ -1,0 +1,103 @@
+"""End-to-end runs of the `rbm` driver on the GPU (main.cpp:381-717
+semantics): training writes the <name>.log CSV (logger.cpp:31-74) and the
+<name>.rbm checkpoint (rbm_base.cpp:219-288, eigen_fstream.cpp:25-51); a
+second run resumes from the checkpoint's update counter; --evaluate prints
+the reference's summary lines.  Energies are checked against the Kitaev
+2x2 exact ground energy (ED, -0.8660254037844461 per site) only as bounds
+(a few SR epochs from random weights).
+"""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RBM = os.path.join(ROOT, "rbm-nqs_amd", "lib", "rbm")
+
+INI = """
+name = {name}
+n_epochs = {epochs}
+noprogress = true
+[model]
+type = kitaev
+n_cells = 2
+J = -1
+symmetry = 0.5
+[rbm]
+type = basic
+alpha = 1
+weights = 0.01
+[sampler]
+n_samples = 2000
+metropolis.n_chains = 100
+[optimizer]
+learning_rate = 0.05
+"""
+
+
+def _run(args, cwd):
+    print("[gpu-test driver]", " ".join(args), flush=True)
+    p = subprocess.run([RBM] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    return p.returncode, p.stdout
+
+
+def _read_rbm(path, N, na):
+    with open(path, "rb") as f:
+        data = f.read()
+    off, mats = 0, []
+    for rows, cols in ((N, na), (na, 1), (N, 1)):
+        r, c = struct.unpack_from("<QQ", data, off)
+        assert (r, c) == (rows, cols)
+        off += 16
+        m = np.frombuffer(data, dtype=np.complex128, count=r * c, offset=off)
+        mats.append(m.reshape(c, r).T)  # column-major
+        off += 16 * r * c
+    (nu,) = struct.unpack_from("<Q", data, off)
+    return mats, nu
+
+
+def test_driver_train_resume_evaluate(tmp_path):
+    d = str(tmp_path)
+    with open(os.path.join(d, "run.ini"), "w") as f:
+        f.write(INI.format(name="k2", epochs=30))
+    rc, out = _run(["-i", "run.ini", "--train"], d)
+    assert rc == 0, out
+    log = open(os.path.join(d, "k2.log")).read().splitlines()
+    assert log[0] == "# Energy, Energy Stddev, AccetpanceRate"
+    rows = np.array([[float(x) for x in l.split(",")] for l in log[1:]])
+    assert rows.shape == (30, 3)
+    # variational energies stay above the exact ground energy and drop
+    assert np.all(rows[:, 0] > -0.8660254037844461 - 5 * rows[:, 1].max())
+    assert rows[-5:, 0].mean() < rows[:5, 0].mean()
+    (W, h, v), nu = _read_rbm(os.path.join(d, "k2.rbm"), 8, 8)
+    assert nu == 30 and np.all(np.isfinite(W))
+    # resume: n_epochs 40 continues from update 30 (10 more lines appended)
+    with open(os.path.join(d, "run.ini"), "w") as f:
+        f.write(INI.format(name="k2", epochs=40))
+    rc, out = _run(["-i", "run.ini", "--train"], d)
+    assert rc == 0, out
+    assert "Loaded RBM from 'k2.rbm'!" in out
+    log2 = open(os.path.join(d, "k2.log")).read().splitlines()
+    assert len([l for l in log2 if not l.startswith("#")]) == 40
+    _, nu2 = _read_rbm(os.path.join(d, "k2.rbm"), 8, 8)
+    assert nu2 == 40
+    # evaluate: hexagon plaquettes, variance, std, tau, acceptance, energy
+    rc, out = _run(["-i", "run.ini", "--evaluate", "--sampler.eval_samples=4000"], d)
+    assert rc == 0, out
+    for key in ("Hex0:", "Var:", "Std:", "Tau:", "Acc:", "E:"):
+        assert key in out, out
+    e = float(re.search(r"E:\s+(\S+)", out).group(1))
+    assert -0.8660254037844461 - 0.05 < e < 0
+
+
+def test_driver_exact_energy_and_errors(tmp_path):
+    rc, out = _run(["--exact_energy", "--model.n_cells=3"], str(tmp_path))
+    assert rc == 0 and "Exact Energy: -0.79397236789" in out
+    rc, out = _run(["--train", "--sampler.type=full"], str(tmp_path))
+    assert rc != 0
