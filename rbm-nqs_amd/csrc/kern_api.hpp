@@ -187,7 +187,7 @@ struct TrainArgs {
 };
 hipError_t launch_train_grad(const TrainArgs& a, hipStream_t st);
 
-// structured (basic RBM) S.x / moments, FP32 MFMA; see kern_struct.hpp
+// structured (basic RBM) S.x / moments, bf16 MFMA at FP32 accuracy; see kern_struct.hpp
 struct StructArgs {
     int64_t S;
     int N, M, P, ld, nw, n_groups, mode;  // mode 0: S.x, 1: y = E_loc, 2: y = 1
@@ -195,11 +195,14 @@ struct StructArgs {
     const cplx<float>* T;
     const double* obs;
     const double* x;
-    cplx<float>* part;  // nblocks x ld: [z_v | z_h | Z | sum |t|^2]
+    cplx<float>* part;  // nblocks x ld: [z_v | z_h | Z | sum |t|^2 (mode 2)]
     const CgStatus* status;
 };
 // LDS bytes of k_struct for padded sizes, 0 if no instance covers N, M
 size_t struct_lds_bytes(int N, int M);
+// resident blocks of k_struct per CU (0 if no instance covers N, M) and the
+// waves of one block
+int struct_blocks_per_cu(int N, int M, int* waves_per_block);
 // e0 / e1 (optional): events recorded at the start and end of the dispatch
 hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st, hipEvent_t e0 = nullptr,
                          hipEvent_t e1 = nullptr);

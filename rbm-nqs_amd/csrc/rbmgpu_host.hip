@@ -804,9 +804,11 @@ int ensure_sx(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
     int64_t blocks, row;
     if (c->structured) {
-        // tiles of 64 samples; two resident blocks per CU walk them
-        int64_t tiles = (c->S + 63) / 64;
-        blocks = std::min<int64_t>(tiles, 2 * (int64_t)c->n_cu);
+        // chunks of 16 samples, one per wave step; the resident waves walk them
+        int64_t chunks = (c->S + 15) / 16;
+        int wpb = 4;
+        int64_t bpc = std::max(struct_blocks_per_cu(c->N, c->na, &wpb), 1);
+        blocks = std::min<int64_t>((chunks + wpb - 1) / wpb, bpc * c->n_cu);
         row = c->ld_s;
     } else {
         if (c->ld > sx_max_params()) return fail(c, "too many parameters for the S.x kernel");
@@ -1147,7 +1149,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     {
         size_t sl = struct_lds_bytes(c->N, c->na);
         c->structured = !c->keep_o && c->nsymm == 1 && c->nvb == c->N && c->pf_rows == 0 &&
-                        c->prec == RBMGPU_FP32 && sl > 0 && sl <= c->lds_limit;
+                        c->prec == RBMGPU_FP32 && c->nw == 1 && sl > 0 && sl <= c->lds_limit;
         c->ld_s = c->P + c->na + ((c->P + c->na) & 1);
     }
     c->params.assign(c->P, 0.);
