@@ -25,7 +25,6 @@ import numpy as np  # noqa: E402
 
 METRIC = "samples/s and SR epoch ms at fixed energy error, plain vs learned proposal"
 HBM_PEAK_GBS = 8000.0
-FP32_MFMA_PEAK_TFS = 157.3
 
 WORKLOAD = dict(model="toric", n_cells=4, rbm="basic", alpha=2, chains=4096, samples_per_chain=64,
                 warmup=100, bond_flips=0.5, sr_max_iterations=100, precision=32, weights_std=0.05)
@@ -127,25 +126,30 @@ def cpu_baseline():
 # matvecs), each scaled by the factor a plain 1 GiB device copy gives
 # (bench.py --copy-calibration 1024: FETCH_SIZE reads 524302 KiB for
 # 1048576 KiB read -> x2.0; WRITE_SIZE reads 1048576 KiB -> x1.0):
-# 62826 KiB x 2.0 + 8068 KiB x 1.0
-STRUCT_TRAFFIC_BYTES = int((62826.4 * 1048576 / 524302.3 + 8067.8) * 1024)
+# 61817 KiB x 2.0 + 3968 KiB x 1.0
+STRUCT_TRAFFIC_BYTES = int((61816.8 * 1048576 / 524302.4 + 3968.4) * 1024)
 
 
 def sx_roofline(g, m, S, sx_ms):
     """Roofline of one S.x of the CG.
 
-    Structured (basic RBM, FP32): k_struct is two real-by-complex GEMMs per
-    tile, G = Sigma X and Z = Sigma^T diag(y) conj(T), 8 S N M flops, plus
-    the O(S M) y / column-sum work (26 S M + 8 S N); MFMA bound against the
-    dense FP32 MFMA peak.  Materialised O: one pass over O (P x S complex
-    FP32), HBM bound."""
+    Structured (basic RBM, FP32): k_struct reads T (S x M complex FP32), one
+    state word per sample and x once per launch -- the algorithmic bytes,
+    S (8 M + 8) + 16 P.  Its two GEMMs (G = Sigma X, Z = Sigma^T diag(y)
+    conj(T): 8 S N M flops, plus 26 S M + 8 S N for y and the column sums)
+    run on bf16 MFMA with exact three-term splits (three MFMAs per product
+    at 16x the FP32 MFMA rate), so HBM is the binding roofline;
+    `fp32_equiv_tflops` is the FP32 flop rate of the same launch.
+    Materialised O: one pass over O (P x S complex FP32), HBM bound."""
     M = (m.P - m.N) // (m.N + 1)  # P = N + M + N M for the basic RBM
     if g.structured():
+        nbytes = S * (8 * M + 8) + 16 * m.P
         flops = 8 * S * m.N * M + 26 * S * M + 8 * S * m.N
-        achieved = flops / (sx_ms * 1e-3) / 1e12 if sx_ms > 0 else 0.0
+        achieved = nbytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
         return {"kernel": "k_struct (one S.x of the CG: G = Sigma X, Z = Sigma^T diag(y) conj(T))",
-                "bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_MFMA_PEAK_TFS, "traffic": STRUCT_TRAFFIC_BYTES,
+                "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": STRUCT_TRAFFIC_BYTES, "algorithmic_bytes": nbytes,
+                "fp32_equiv_tflops": flops / (sx_ms * 1e-3) / 1e12 if sx_ms > 0 else 0.0,
                 "flops_per_launch": flops, "launch_ms": sx_ms}
     o_bytes = g.P * S * 8
     achieved = o_bytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0

@@ -106,6 +106,10 @@ template <typename R, int L, int K>
 struct Units {
     int a[K], s[K];  // alpha and symmetry index per unit, a < 0: padding
     cplx<R> th[K], tn[K];
+    // FP32: cosh and sinh of the last ratio12 flip set's delta per unit and
+    // the unit's ratio factor cosh(theta + delta) / cosh(theta), kept for
+    // accept12's tanh update
+    cplx<R> kc[K], ks[K], kr[K];
 
     __device__ void init(const Dims& d, int ln) {
 #pragma unroll
@@ -133,8 +137,7 @@ struct Units {
         }
     }
     // lane-partial product of cosh(theta')/cosh(theta) for one or two flips
-    __device__ void ratio12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1, cplx<R>& p,
-                            int& pe) const {
+    __device__ void ratio12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1, cplx<R>& p, int& pe) {
         p = {1, 0};
         pe = 0;
 #pragma unroll
@@ -151,8 +154,23 @@ struct Units {
                 sh = sh * c1 + ch * sn1;
                 ch = nch;
             }
-            p = p * (ch + tn[u] * sh);
+            const cplx<R> r = ch + tn[u] * sh;
+            if constexpr (sizeof(R) == 4) {
+                kc[u] = ch;
+                ks[u] = sh;
+                kr[u] = r;
+            }
+            p = p * r;
             if (sizeof(R) == 4 && (u & 3) == 3) renorm(p, pe);
+        }
+    }
+    // FP32 after a sweep: tanh from theta again (the accepted moves update
+    // tanh incrementally, see accept12_w)
+    __device__ void resync() {
+        if constexpr (sizeof(R) == 4) {
+#pragma unroll
+            for (int u = 0; u < K; u++)
+                if (a[u] >= 0) tn[u] = ctanh(th[u]);
         }
     }
     // generic flip set: sites fl[i] for i < 8 with bit i of mask set
@@ -200,6 +218,13 @@ struct Units {
         else
             accept12_w(d, l, l.Wg, f0, s0, f1, s1);
     }
+    // FP64 recomputes tanh(theta) (the chains follow the oracle's state for
+    // state); FP32 updates it from the ratio12 of the same flip set:
+    // tanh(theta + delta) = (t cosh delta + sinh delta) / (cosh delta +
+    // t sinh delta), whose denominator is the unit's ratio factor (a move
+    // is accepted with probability |prod of factors|^2, so a vanishing
+    // factor is almost never divided by); resync() restores tanh(theta)
+    // after every sweep, so rounding builds up over one sweep at most
     __device__ void accept12_w(const Dims& d, const Lds<R>& l, const cplx<R>* W, int f0, R s0, int f1, R s1) {
 #pragma unroll
         for (int u = 0; u < K; u++) {
@@ -210,7 +235,13 @@ struct Units {
                 cplx<R> w1 = W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
                 th[u] = th[u] - (2 * s1) * w1;
             }
-            tn[u] = ctanh(th[u]);
+            if constexpr (sizeof(R) == 4) {
+                const cplx<R> num = tn[u] * kc[u] + ks[u], r = kr[u];
+                const R inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
+                tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
+            } else {
+                tn[u] = ctanh(th[u]);
+            }
         }
     }
 };
@@ -397,6 +428,7 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
                     if (a.pf_refresh > 0 && acc % (uint32_t)a.pf_refresh == 0) pf_fresh(l.pf, st, ln, pv);
             }
         }
+        U.resync();
         if (step >= a.warmup && (step - a.warmup) % a.step == 0) {
             int64_t smp = off + k;
             if (ln == 0)

@@ -2,8 +2,6 @@
 // FP32 accuracy).
 #include <hip/hip_ext.h>
 
-#include <algorithm>
-
 #include "kern_struct.hpp"
 
 namespace rbmgpu {
@@ -31,7 +29,7 @@ static bool pick(int N, int M, int& ct, int& rt) {
 
 static size_t lds_for(int ct, int rt) {
 #define X(C_, R_) \
-    if (ct == C_ && rt == R_) return std::max(StructCfg<C_, R_, 1>::bytes, StructCfg<C_, R_, 2>::bytes);
+    if (ct == C_ && rt == R_) return StructCfg<C_, R_>::bytes;
     STRUCT_SHAPES(X)
 #undef X
     return 0;
@@ -43,20 +41,10 @@ size_t struct_lds_bytes(int N, int M) {
     return lds_for(ct, rt);
 }
 
-#include <cstdlib>
-// samples per wave step: 16 TA (experiment switch, read once)
-static int tiles() {
-    static int t = [] {
-        const char* e = std::getenv("RBMGPU_STRUCT_TILES");
-        return (e && e[0] == '2') ? 2 : 1;
-    }();
-    return t;
-}
-
-template <int CT, int RT, int TA>
+template <int CT, int RT>
 static int occupancy() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_struct<CT, RT, TA>, StructCfg<CT, RT, TA>::THREADS,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_struct<CT, RT>, StructCfg<CT, RT>::THREADS,
                                                      lds_for(CT, RT)) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
@@ -67,28 +55,28 @@ int struct_blocks_per_cu(int N, int M, int* waves_per_block) {
     if (!pick(N, M, ct, rt)) return 0;
 #define X(C_, R_)                                                                               \
     if (ct == C_ && rt == R_) {                                                                 \
-        *waves_per_block = tiles() == 2 ? StructCfg<C_, R_, 2>::NWB : StructCfg<C_, R_, 1>::NWB; \
-        return tiles() == 2 ? occupancy<C_, R_, 2>() : occupancy<C_, R_, 1>();                  \
+        *waves_per_block = StructCfg<C_, R_>::NWB;                                           \
+        return occupancy<C_, R_>();                                                          \
     }
     STRUCT_SHAPES(X)
 #undef X
     return 0;
 }
 
-template <int CT, int RT, int TA>
+template <int CT, int RT>
 static hipError_t one(const StructArgs& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     size_t lds = lds_for(CT, RT);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_struct<CT, RT, TA>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_struct<CT, RT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     // with events: timestamps of this dispatch itself (start and end of the kernel)
     if (e0 || e1)
-        hipExtLaunchKernelGGL((k_struct<CT, RT, TA>), dim3(blocks), dim3(StructCfg<CT, RT, TA>::THREADS), (uint32_t)lds,
+        hipExtLaunchKernelGGL((k_struct<CT, RT>), dim3(blocks), dim3(StructCfg<CT, RT>::THREADS), (uint32_t)lds,
                               st, e0, e1, 0u, a);
     else
-        hipLaunchKernelGGL((k_struct<CT, RT, TA>), dim3(blocks), dim3(StructCfg<CT, RT, TA>::THREADS), lds, st, a);
+        hipLaunchKernelGGL((k_struct<CT, RT>), dim3(blocks), dim3(StructCfg<CT, RT>::THREADS), lds, st, a);
     return hipGetLastError();
 }
 
@@ -96,7 +84,7 @@ hipError_t launch_struct(const StructArgs& a, int blocks, hipStream_t st, hipEve
     int ct, rt;
     if (!pick(a.N, a.M, ct, rt)) return hipErrorInvalidValue;
 #define X(C_, R_) \
-    if (ct == C_ && rt == R_) return tiles() == 2 ? one<C_, R_, 2>(a, blocks, st, e0, e1) : one<C_, R_, 1>(a, blocks, st, e0, e1);
+    if (ct == C_ && rt == R_) return one<C_, R_>(a, blocks, st, e0, e1);
     STRUCT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
