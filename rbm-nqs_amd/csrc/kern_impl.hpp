@@ -106,9 +106,11 @@ template <typename R, int L, int K>
 struct Units {
     int a[K], s[K];  // alpha and symmetry index per unit, a < 0: padding
     cplx<R> th[K], tn[K];
-    // FP32: cosh and sinh of the last ratio12 flip set's delta per unit and
-    // the unit's ratio factor cosh(theta + delta) / cosh(theta), kept for
-    // accept12's tanh update
+    // FP32 (K <= 8; more units would not fit in the registers): cosh and
+    // sinh of the last ratio12 flip set's delta per unit and the unit's
+    // ratio factor cosh(theta + delta) / cosh(theta), kept for accept12's
+    // tanh update
+    static constexpr bool kMob = sizeof(R) == 4 && K <= 8;
     cplx<R> kc[K], ks[K], kr[K];
 
     __device__ void init(const Dims& d, int ln) {
@@ -155,7 +157,7 @@ struct Units {
                 ch = nch;
             }
             const cplx<R> r = ch + tn[u] * sh;
-            if constexpr (sizeof(R) == 4) {
+            if constexpr (kMob) {
                 kc[u] = ch;
                 ks[u] = sh;
                 kr[u] = r;
@@ -164,17 +166,17 @@ struct Units {
             if (sizeof(R) == 4 && (u & 3) == 3) renorm(p, pe);
         }
     }
-    // FP32 after a sweep: tanh from theta again (the accepted moves update
-    // tanh incrementally, see accept12_w)
+    // FP32 (kMob) after a sweep: tanh from theta again (the accepted moves
+    // update tanh incrementally, see accept12_w)
     __device__ void resync() {
-        if constexpr (sizeof(R) == 4) {
+        if constexpr (kMob) {
 #pragma unroll
             for (int u = 0; u < K; u++)
                 if (a[u] >= 0) tn[u] = ctanh(th[u]);
         }
     }
     // generic flip set: sites fl[i] for i < 8 with bit i of mask set
-    __device__ void ratio_mask(const Dims& d, const Lds<R>& l, const int* fl, const R* sg, int mask,
+    __device__ __forceinline__ void ratio_mask(const Dims& d, const Lds<R>& l, const int* fl, const R* sg, int mask,
                                cplx<R>& p, int& pe) const {
         p = {1, 0};
         pe = 0;
@@ -219,7 +221,7 @@ struct Units {
             accept12_w(d, l, l.Wg, f0, s0, f1, s1);
     }
     // FP64 recomputes tanh(theta) (the chains follow the oracle's state for
-    // state); FP32 updates it from the ratio12 of the same flip set:
+    // state); FP32 (kMob) updates it from the ratio12 of the same flip set:
     // tanh(theta + delta) = (t cosh delta + sinh delta) / (cosh delta +
     // t sinh delta), whose denominator is the unit's ratio factor (a move
     // is accepted with probability |prod of factors|^2, so a vanishing
@@ -235,7 +237,7 @@ struct Units {
                 cplx<R> w1 = W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
                 th[u] = th[u] - (2 * s1) * w1;
             }
-            if constexpr (sizeof(R) == 4) {
+            if constexpr (kMob) {
                 const cplx<R> num = tn[u] * kc[u] + ks[u], r = kr[u];
                 const R inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
                 tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
@@ -250,7 +252,7 @@ struct Units {
 // visible-bias factor exp(-2 sum sigma_f c_f) (rbm_base.cpp:306-318) times
 // the group product of the per-unit cosh ratios
 template <typename R, int L, int K, bool PF>
-__device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
+__device__ __forceinline__ cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
                                           const cplx<R> (&pv)[16], const uint64_t* st, int ln, const int* fl,
                                           const R* sg, int mask) {
     cplx<R> bias{0, 0};
@@ -318,9 +320,204 @@ __device__ inline void eval_obs_t(const Dims& d, const Lds<R>& l, const Units<R,
         out[2 * g + 1] = acc.im;
     }
 }
+
+// FP32 local observables (no Pfaffian factor), same terms and sums as
+// eval_obs_t, arranged for latency: the lanes first resolve the terms in
+// parallel (lane ln: terms ln + j L, j < kObsWalk -- sites, local
+// configuration, entry range), then the terms are taken in order in
+// batches of up to kObsBatch terms of one group, each reading its owner
+// lane's result by a lane shuffle; the lane-partial cosh ratios of a
+// batch's off-diagonal entries are reduced over the group together
+// (interleaved butterflies) and multiplied by the visible-bias factor
+// exp(-2 sum sigma_f c_f), computed with v_exp_f32 and sincos_f.  A term
+// with more than one off-diagonal entry finishes the others one by one.
+// Used for K <= 2 units per lane.
+constexpr int kObsWalk = 4, kObsBatch = 4;
+
+template <int L, int B, int O = 1>
+__device__ inline void group_cprod_batch(cplx<float> (&m)[B], int (&e)[B]) {
+    if constexpr (O < L) {
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            cplx<float> q{partner<O>(m[b].re), partner<O>(m[b].im)};
+            int qe = partner<O>(e[b]);
+            m[b] = m[b] * q;
+            e[b] += qe;
+        }
+        group_cprod_batch<L, B, 2 * O>(m, e);
+    }
+}
+
+template <int L, int K, bool TL>
+__device__ inline void eval_obs_batch(const Dims& d, const Lds<float>& l, const Units<float, L, K>& U,
+                                      const cplx<float> (&pv)[16], const uint64_t* st, int ln, double* out) {
+    const Terms& T = TL ? l.Tl : l.Tg;
+    constexpr int J = kObsWalk, B = kObsBatch;
+    const int nt = T.n_terms;
+    // ---- walk: site bytes (packed), entry range of terms ln + j L
+    uint32_t ws0[J], ws1[J];
+    int we0[J], we1[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int ti = ln + j * L;
+        ws0[j] = ws1[j] = 0;
+        we0[j] = we1[j] = 0;
+        if (ti < nt) {
+            const int k = T.k[ti];
+            int loc = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (i < k) {
+                    const uint32_t f = T.sites[ti * 8 + i];
+                    if (i < 4) ws0[j] |= f << (8 * i);
+                    else ws1[j] |= f << (8 * (i - 4));
+                    loc |= bitof(st, (int)f) << i;
+                }
+            }
+            const int base = T.rowbase[ti] + loc;
+            we0[j] = T.rowptr[base];
+            we1[j] = T.rowptr[base + 1];
+        }
+    }
+    // ---- terms in order, by batches of one group
+    struct Slot {
+        cplx<float> p, bias;  // lane-partial ratio (mantissa p 2^pe), visible-bias exponent
+        int pe;
+        cplx<double> v;  // matrix element
+        bool used;
+    };
+    cplx<double> acc{0, 0};
+    int ti = 0;
+    int g = nt > 0 ? T.group[0] : 0;
+    // next term of group g into slot S (diagonal entries go straight to acc)
+    auto fill = [&](Slot& S) __attribute__((always_inline)) {
+        S.p = {1.f, 0.f};
+        S.pe = 0;
+        S.bias = {0.f, 0.f};
+        S.v = {0, 0};
+        S.used = false;
+        if (ti >= nt || T.group[ti] != g) return;
+        const int t = ti++, j = t / L, src = t % L;
+        uint32_t s0 = ws0[0], s1 = ws1[0];
+        int e0 = we0[0], e1 = we1[0];
+#pragma unroll
+        for (int jj = 1; jj < J; jj++)
+            if (j == jj) {
+                s0 = ws0[jj];
+                s1 = ws1[jj];
+                e0 = we0[jj];
+                e1 = we1[jj];
+            }
+        s0 = (uint32_t)__shfl((int)s0, src, L);
+        s1 = (uint32_t)__shfl((int)s1, src, L);
+        e0 = __shfl(e0, src, L);
+        e1 = __shfl(e1, src, L);
+        for (int e = e0; e < e1; e++) {
+            const int mask = T.mask[e];
+            const cplx<double> v{T.val[2 * e], T.val[2 * e + 1]};
+            if (mask == 0) {
+                acc = acc + v;
+                continue;
+            }
+            // lane-partial product of cosh(theta')/cosh(theta) over the
+            // flipped sites (set bits of mask, in order; the loop over the
+            // bits keeps the table loads to the sites that are flipped)
+            cplx<float> ch[K], sh[K];
+#pragma unroll
+            for (int u = 0; u < K; u++) {
+                ch[u] = {1.f, 0.f};
+                sh[u] = {0.f, 0.f};
+            }
+            cplx<float> bias{0.f, 0.f};
+            for (int m = mask; m; m &= m - 1) {
+                const int i = __builtin_ctz(m);
+                const int f = (int)(((i < 4 ? s0 : s1) >> (8 * (i & 3))) & 0xffu);
+                const float sgn = bitof(st, f) ? -1.f : 1.f;
+                bias = bias + (-2 * sgn) * l.rb[f];
+#pragma unroll
+                for (int u = 0; u < K; u++) {
+                    if (U.a[u] < 0) continue;
+                    const int ii = kidx(d, l.P, U.s[u], f) * d.n_alpha + U.a[u];
+                    const cplx<float> c = l.C[ii], sn = (-sgn) * l.S[ii];
+                    const cplx<float> nch = ch[u] * c + sh[u] * sn;
+                    sh[u] = sh[u] * c + ch[u] * sn;
+                    ch[u] = nch;
+                }
+            }
+            cplx<float> p{1.f, 0.f};
+            int pe = 0;
+#pragma unroll
+            for (int u = 0; u < K; u++) {
+                if (U.a[u] < 0) continue;
+                p = p * (ch[u] + U.tn[u] * sh[u]);
+                if ((u & 3) == 3) renorm(p, pe);
+            }
+            if (!S.used) {
+                S.p = p;
+                S.pe = pe;
+                S.bias = bias;
+                S.v = v;
+                S.used = true;
+            } else {  // a second off-diagonal entry of the term
+                group_cprod<L>(p, pe);
+                float sn, cs;
+                sincos_f(bias.im, sn, cs);
+                const float ex = __expf(bias.re);
+                const cplx<float> r = cplx<float>{ex * cs, ex * sn} * p;
+                const double sc = ::ldexp(1.0, pe);
+                acc = acc + v * cplx<double>{(double)r.re * sc, (double)r.im * sc};
+            }
+        }
+    };
+    static_assert(B == 4, "four slots per batch");
+    while (ti < nt) {
+        Slot s0, s1, s2, s3;
+        fill(s0);
+        fill(s1);
+        fill(s2);
+        fill(s3);
+        cplx<float> bp[B] = {s0.p, s1.p, s2.p, s3.p};
+        int bpe[B] = {s0.pe, s1.pe, s2.pe, s3.pe};
+#pragma unroll
+        for (int b = 0; b < B; b++) renorm(bp[b], bpe[b]);
+        group_cprod_batch<L, B>(bp, bpe);
+        auto finish = [&](const Slot& S, cplx<float> p, int pe) __attribute__((always_inline)) {
+            if (!S.used) return;
+            float sn, cs;
+            sincos_f(S.bias.im, sn, cs);
+            const float ex = __expf(S.bias.re);
+            const cplx<float> r = cplx<float>{ex * cs, ex * sn} * p;
+            const double sc = ::ldexp(1.0, pe);
+            acc = acc + S.v * cplx<double>{(double)r.re * sc, (double)r.im * sc};
+        };
+        finish(s0, bp[0], bpe[0]);
+        finish(s1, bp[1], bpe[1]);
+        finish(s2, bp[2], bpe[2]);
+        finish(s3, bp[3], bpe[3]);
+        if (ti >= nt || T.group[ti] != g) {
+            if (ln == 0) {
+                out[2 * g] = acc.re;
+                out[2 * g + 1] = acc.im;
+            }
+            acc = {0, 0};
+            if (ti < nt) g = T.group[ti];
+        }
+    }
+}
+
 template <typename R, int L, int K, bool PF>
 __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
                                 const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
+    // (with more units per lane the batch would not fit in the registers)
+    if constexpr (sizeof(R) == 4 && !PF && K <= 2) {
+        if (d.n_terms <= kObsWalk * L) {
+            if (d.t_lds)
+                eval_obs_batch<L, K, true>(d, l, U, pv, st, ln, out);
+            else
+                eval_obs_batch<L, K, false>(d, l, U, pv, st, ln, out);
+            return;
+        }
+    }
     if (d.t_lds)
         eval_obs_t<R, L, K, PF, true>(d, l, U, pv, st, ln, out);
     else
