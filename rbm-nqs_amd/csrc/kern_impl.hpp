@@ -106,11 +106,11 @@ template <typename R, int L, int K>
 struct Units {
     int a[K], s[K];  // alpha and symmetry index per unit, a < 0: padding
     cplx<R> th[K], tn[K];
-    // FP32 (K <= 8; more units would not fit in the registers): cosh and
+    // FP32 (K <= 4; more units would cost occupancy): cosh and
     // sinh of the last ratio12 flip set's delta per unit and the unit's
     // ratio factor cosh(theta + delta) / cosh(theta), kept for accept12's
     // tanh update
-    static constexpr bool kMob = sizeof(R) == 4 && K <= 8;
+    static constexpr bool kMob = sizeof(R) == 4 && K <= 4;
     cplx<R> kc[K], ks[K], kr[K];
 
     __device__ void init(const Dims& d, int ln) {
@@ -176,7 +176,7 @@ struct Units {
         }
     }
     // generic flip set: sites fl[i] for i < 8 with bit i of mask set
-    __device__ __forceinline__ void ratio_mask(const Dims& d, const Lds<R>& l, const int* fl, const R* sg, int mask,
+    __device__ void ratio_mask(const Dims& d, const Lds<R>& l, const int* fl, const R* sg, int mask,
                                cplx<R>& p, int& pe) const {
         p = {1, 0};
         pe = 0;
@@ -252,7 +252,7 @@ struct Units {
 // visible-bias factor exp(-2 sum sigma_f c_f) (rbm_base.cpp:306-318) times
 // the group product of the per-unit cosh ratios
 template <typename R, int L, int K, bool PF>
-__device__ __forceinline__ cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
+__device__ inline cplx<double> term_ratio(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
                                           const cplx<R> (&pv)[16], const uint64_t* st, int ln, const int* fl,
                                           const R* sg, int mask) {
     cplx<R> bias{0, 0};
