@@ -103,6 +103,42 @@ REDUCE_CASES = [("C1", 64, False), ("C1", 32, False), ("C1", 32, True), ("C2", 6
                 ("C2", 32, False), ("C2", 32, True), ("C3", 64, False), ("C3", 32, False)]
 
 
+# every tile shape of the structured S.x: (model, n_cells, alpha) -> (CT, RT)
+# = hidden / spin tiles of 16; (8, x) loads its chunks into registers, the
+# others through the LDS chunk buffers
+STRUCT_SHAPES = [("kitaev", 2, 1, "1x1"), ("kitaev", 2, 4, "2x1"), ("kitaev", 3, 1, "2x2"),
+                 ("kitaev", 2, 8, "4x1"), ("toric", 4, 2, "4x2"), ("toric", 5, 1, "4x4"),
+                 ("kitaev", 2, 16, "8x1"), ("kitaev", 4, 4, "8x2")]
+
+
+@pytest.mark.parametrize("model,n,alpha,shape", STRUCT_SHAPES)
+@pytest.mark.parametrize("S", [200, 5])
+def test_structured_shapes(model, n, alpha, shape, S):
+    """bf16x3 structured S.x and moments at every tile shape, for a sample
+    count that is not a multiple of the 16-sample chunk and one smaller
+    than a chunk.  The split products are exact, so the FP32 error comes
+    from the FP32 tanh rows (measured <= 1.1e-4 of max at toric 5x5);
+    stated tolerance 5e-4 of max, as for the materialised FP32 path."""
+    spec = ini(model, n, alpha=alpha, extra="[sampler]\nn_samples = 200\nmetropolis.n_chains = 8\n")
+    m, o, g, rng = _setup(spec, 32, std=0.1, seed=3)
+    assert g.structured(), shape
+    states = po.random_states(rng, S, m.N)
+    g.load_samples(states)
+    e = o.eloc(states)
+    O = o.derivative(states)
+    ref = o.aggregate(e, O)
+    got = g.reduce()
+    tol = 5e-4
+    for k in ("o_mean", "eo"):
+        assert np.max(np.abs(got[k] - ref[k])) <= tol * (1 + np.abs(ref[k]).max()), k
+    assert np.max(np.abs(got["o_var"] - ref["o_var"])) <= tol * (1 + np.abs(ref["o_var"]).max())
+    x = rng.normal(size=m.P) + 1j * rng.normal(size=m.P)
+    r1, r2, rd = 0.9, 9e-4, 1e-2
+    y = g.apply_S(x, r1, r2, rd)
+    y_ref = po.apply_S(O, ref["o_mean"], ref["o_var"], r1, r2, rd, m.n_neural, x)
+    assert np.max(np.abs(y - y_ref)) <= tol * np.abs(y_ref).max(), np.max(np.abs(y - y_ref)) / np.abs(y_ref).max()
+
+
 @pytest.mark.parametrize("name,prec,keep_o", REDUCE_CASES)
 def test_reduce_S_and_cg(name, prec, keep_o):
     m, o, g, rng = _setup(SPECS[name], prec, std=0.2 if prec == 64 else 0.1, keep_o=keep_o)
