@@ -331,7 +331,7 @@ __device__ inline void eval_obs_t(const Dims& d, const Lds<R>& l, const Units<R,
 // (interleaved butterflies) and multiplied by the visible-bias factor
 // exp(-2 sum sigma_f c_f), computed with v_exp_f32 and sincos_f.  A term
 // with more than one off-diagonal entry finishes the others one by one.
-// Used for K <= 2 units per lane.
+// Used for K <= 2 units per lane, term tables in LDS and the basic RBM.
 constexpr int kObsWalk = 4, kObsBatch = 4;
 
 template <int L, int B, int O = 1>
@@ -348,7 +348,7 @@ __device__ inline void group_cprod_batch(cplx<float> (&m)[B], int (&e)[B]) {
     }
 }
 
-template <int L, int K, bool TL>
+template <int L, int K, bool TL, bool SYM1>
 __device__ inline void eval_obs_batch(const Dims& d, const Lds<float>& l, const Units<float, L, K>& U,
                                       const cplx<float> (&pv)[16], const uint64_t* st, int ln, double* out) {
     const Terms& T = TL ? l.Tl : l.Tg;
@@ -433,14 +433,25 @@ __device__ inline void eval_obs_batch(const Dims& d, const Lds<float>& l, const 
                 const int i = __builtin_ctz(m);
                 const int f = (int)(((i < 4 ? s0 : s1) >> (8 * (i & 3))) & 0xffu);
                 const float sgn = bitof(st, f) ? -1.f : 1.f;
-                bias = bias + (-2 * sgn) * l.rb[f];
+                // all loads of the site at once (one wait): its bias and the
+                // units' table rows (basic RBM: row f; padded units read the
+                // row of alpha 0 and are not used)
+                const cplx<float> rbv = l.rb[f];
+                cplx<float> c[K], sn[K];
+#pragma unroll
+                for (int u = 0; u < K; u++) {
+                    const int row = SYM1 ? f : (int)l.P[U.s[u] * d.N + f];
+                    const int ii = row * d.n_alpha + max(U.a[u], 0);
+                    c[u] = l.C[ii];
+                    sn[u] = l.S[ii];
+                }
+                bias = bias + (-2 * sgn) * rbv;
 #pragma unroll
                 for (int u = 0; u < K; u++) {
                     if (U.a[u] < 0) continue;
-                    const int ii = kidx(d, l.P, U.s[u], f) * d.n_alpha + U.a[u];
-                    const cplx<float> c = l.C[ii], sn = (-sgn) * l.S[ii];
-                    const cplx<float> nch = ch[u] * c + sh[u] * sn;
-                    sh[u] = sh[u] * c + ch[u] * sn;
+                    const cplx<float> snu = (-sgn) * sn[u];
+                    const cplx<float> nch = ch[u] * c[u] + sh[u] * snu;
+                    sh[u] = sh[u] * c[u] + ch[u] * snu;
                     ch[u] = nch;
                 }
             }
@@ -508,13 +519,11 @@ __device__ inline void eval_obs_batch(const Dims& d, const Lds<float>& l, const 
 template <typename R, int L, int K, bool PF>
 __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
                                 const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
-    // (with more units per lane the batch would not fit in the registers)
+    // (with more units per lane the batch would not fit in the registers;
+    // one inlined copy only: term tables in LDS, one table row per site)
     if constexpr (sizeof(R) == 4 && !PF && K <= 2) {
-        if (d.n_terms <= kObsWalk * L) {
-            if (d.t_lds)
-                eval_obs_batch<L, K, true>(d, l, U, pv, st, ln, out);
-            else
-                eval_obs_batch<L, K, false>(d, l, U, pv, st, ln, out);
+        if (d.n_terms <= kObsWalk * L && d.t_lds && d.n_symm == 1) {
+            eval_obs_batch<L, K, true, true>(d, l, U, pv, st, ln, out);
             return;
         }
     }
