@@ -138,8 +138,8 @@ struct Units {
             tn[u] = ctanh(acc);
         }
     }
-    // lane-partial product of cosh(theta')/cosh(theta) for one or two flips
-    __device__ void ratio12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1, cplx<R>& p, int& pe) {
+    __device__ void ratio12_units(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1, cplx<R>& p,
+                                  int& pe) {
         p = {1, 0};
         pe = 0;
 #pragma unroll
@@ -154,6 +154,65 @@ struct Units {
                 cplx<R> sn1 = (-s1) * l.S[i1];
                 cplx<R> nch = ch * c1 + sh * sn1;
                 sh = sh * c1 + ch * sn1;
+                ch = nch;
+            }
+            const cplx<R> r = ch + tn[u] * sh;
+            if constexpr (kMob) {
+                kc[u] = ch;
+                ks[u] = sh;
+                kr[u] = r;
+            }
+            p = p * r;
+            if (sizeof(R) == 4 && (u & 3) == 3) renorm(p, pe);
+        }
+    }
+    // table rows of site f for the lane's units (symmetric RBM: row of the
+    // permuted site; one uniform branch, the loads of all units together)
+    __device__ void rows(const Dims& d, const Lds<R>& l, int f, int (&r)[K]) const {
+        if (d.n_symm == 1) {
+#pragma unroll
+            for (int u = 0; u < K; u++) r[u] = f;
+        } else {
+#pragma unroll
+            for (int u = 0; u < K; u++) r[u] = (int)l.P[s[u] * d.N + f];
+        }
+    }
+    // lane-partial product of cosh(theta')/cosh(theta) for one or two flips.
+    // K <= 2: every unit's table entries are loaded before any is used
+    // (padded units, a < 0, read the entries of alpha 0 and are skipped), so
+    // one wait covers them; wider shapes load unit by unit (fewer live
+    // registers: the batched form would cost them a wave per SIMD)
+    __device__ void ratio12(const Dims& d, const Lds<R>& l, int f0, R s0, int f1, R s1, cplx<R>& p, int& pe) {
+        if constexpr (K > 2) {
+            ratio12_units(d, l, f0, s0, f1, s1, p, pe);
+            return;
+        }
+        int r0[K], r1[K];
+        rows(d, l, f0, r0);
+        if (f1 >= 0) rows(d, l, f1, r1);
+        cplx<R> c0[K], q0[K], c1[K], q1[K];
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            const int i0 = r0[u] * d.n_alpha + max(a[u], 0);
+            c0[u] = l.C[i0];
+            q0[u] = l.S[i0];
+            if (f1 >= 0) {
+                const int i1 = r1[u] * d.n_alpha + max(a[u], 0);
+                c1[u] = l.C[i1];
+                q1[u] = l.S[i1];
+            }
+        }
+        p = {1, 0};
+        pe = 0;
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            if (a[u] < 0) continue;
+            cplx<R> ch = c0[u];
+            cplx<R> sh = (-s0) * q0[u];
+            if (f1 >= 0) {
+                cplx<R> sn1 = (-s1) * q1[u];
+                cplx<R> nch = ch * c1[u] + sh * sn1;
+                sh = sh * c1[u] + ch * sn1;
                 ch = nch;
             }
             const cplx<R> r = ch + tn[u] * sh;
@@ -228,22 +287,44 @@ struct Units {
     // factor is almost never divided by); resync() restores tanh(theta)
     // after every sweep, so rounding builds up over one sweep at most
     __device__ void accept12_w(const Dims& d, const Lds<R>& l, const cplx<R>* W, int f0, R s0, int f1, R s1) {
+        if constexpr (K > 2) {  // unit by unit (see ratio12)
+#pragma unroll
+            for (int u = 0; u < K; u++) {
+                if (a[u] < 0) continue;
+                cplx<R> w0 = W[kidx(d, l.P, s[u], f0) * d.n_alpha + a[u]];
+                th[u] = th[u] - (2 * s0) * w0;
+                if (f1 >= 0) {
+                    cplx<R> w1 = W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
+                    th[u] = th[u] - (2 * s1) * w1;
+                }
+                update_tanh(u);
+            }
+            return;
+        }
+        int r0[K], r1[K];
+        rows(d, l, f0, r0);
+        if (f1 >= 0) rows(d, l, f1, r1);
+        cplx<R> w0[K], w1[K];
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            w0[u] = W[r0[u] * d.n_alpha + max(a[u], 0)];
+            if (f1 >= 0) w1[u] = W[r1[u] * d.n_alpha + max(a[u], 0)];
+        }
 #pragma unroll
         for (int u = 0; u < K; u++) {
             if (a[u] < 0) continue;
-            cplx<R> w0 = W[kidx(d, l.P, s[u], f0) * d.n_alpha + a[u]];
-            th[u] = th[u] - (2 * s0) * w0;
-            if (f1 >= 0) {
-                cplx<R> w1 = W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
-                th[u] = th[u] - (2 * s1) * w1;
-            }
-            if constexpr (kMob) {
-                const cplx<R> num = tn[u] * kc[u] + ks[u], r = kr[u];
-                const R inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
-                tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
-            } else {
-                tn[u] = ctanh(th[u]);
-            }
+            th[u] = th[u] - (2 * s0) * w0[u];
+            if (f1 >= 0) th[u] = th[u] - (2 * s1) * w1[u];
+            update_tanh(u);
+        }
+    }
+    __device__ void update_tanh(int u) {
+        if constexpr (kMob) {
+            const cplx<R> num = tn[u] * kc[u] + ks[u], r = kr[u];
+            const R inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
+            tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
+        } else {
+            tn[u] = ctanh(th[u]);
         }
     }
 };
