@@ -112,6 +112,11 @@ struct Units {
     // tanh update
     static constexpr bool kMob = sizeof(R) == 4 && K <= 4;
     cplx<R> kc[K], ks[K], kr[K];
+    // K <= 2, L <= 32 (the Pfaffian shapes have L = 64 and no registers to
+    // spare): W entries of the last ratio12 flip set, loaded with its C
+    // and S entries for accept12
+    static constexpr bool kPreW = K <= 2 && L <= 32;
+    cplx<R> kw0[kPreW ? K : 1], kw1[kPreW ? K : 1];
 
     __device__ void init(const Dims& d, int ln) {
 #pragma unroll
@@ -191,15 +196,18 @@ struct Units {
         rows(d, l, f0, r0);
         if (f1 >= 0) rows(d, l, f1, r1);
         cplx<R> c0[K], q0[K], c1[K], q1[K];
+        const cplx<R>* W = d.w_lds ? l.Wl : l.Wg;
 #pragma unroll
         for (int u = 0; u < K; u++) {
             const int i0 = r0[u] * d.n_alpha + max(a[u], 0);
             c0[u] = l.C[i0];
             q0[u] = l.S[i0];
+            if constexpr (kPreW) kw0[u] = W[i0];
             if (f1 >= 0) {
                 const int i1 = r1[u] * d.n_alpha + max(a[u], 0);
                 c1[u] = l.C[i1];
                 q1[u] = l.S[i1];
+                if constexpr (kPreW) kw1[u] = W[i1];
             }
         }
         p = {1, 0};
@@ -287,7 +295,7 @@ struct Units {
     // factor is almost never divided by); resync() restores tanh(theta)
     // after every sweep, so rounding builds up over one sweep at most
     __device__ void accept12_w(const Dims& d, const Lds<R>& l, const cplx<R>* W, int f0, R s0, int f1, R s1) {
-        if constexpr (K > 2) {  // unit by unit (see ratio12)
+        if constexpr (!kPreW) {  // unit by unit
 #pragma unroll
             for (int u = 0; u < K; u++) {
                 if (a[u] < 0) continue;
@@ -299,23 +307,14 @@ struct Units {
                 }
                 update_tanh(u);
             }
-            return;
-        }
-        int r0[K], r1[K];
-        rows(d, l, f0, r0);
-        if (f1 >= 0) rows(d, l, f1, r1);
-        cplx<R> w0[K], w1[K];
+        } else {  // the W entries came with ratio12 of the same flip set
 #pragma unroll
-        for (int u = 0; u < K; u++) {
-            w0[u] = W[r0[u] * d.n_alpha + max(a[u], 0)];
-            if (f1 >= 0) w1[u] = W[r1[u] * d.n_alpha + max(a[u], 0)];
-        }
-#pragma unroll
-        for (int u = 0; u < K; u++) {
-            if (a[u] < 0) continue;
-            th[u] = th[u] - (2 * s0) * w0[u];
-            if (f1 >= 0) th[u] = th[u] - (2 * s1) * w1[u];
-            update_tanh(u);
+            for (int u = 0; u < K; u++) {
+                if (a[u] < 0) continue;
+                th[u] = th[u] - (2 * s0) * kw0[u];
+                if (f1 >= 0) th[u] = th[u] - (2 * s1) * kw1[u];
+                update_tanh(u);
+            }
         }
     }
     __device__ void update_tanh(int u) {
