@@ -154,9 +154,10 @@ int rbmgpu_n_samples(const rbmgpu_ctx* ctx, int64_t* n);
 
 /* device time of the last launch of each kernel family, from HIP events
  * on the context's stream (no counterpart in the reference): [sample,
- * obs/O assembly, reduce, S.x main kernel + chunk reduction (mean per CG
- * matvec), unused (0), S.x main kernel alone (k_struct or k_sx_partial,
- * mean per matvec)] in ms. */
+ * obs/O assembly, reduce, one CG iteration (S.x and the CG step; mean over
+ * the last rbmgpu_sr_solve, launches queued past convergence included) or
+ * the last rbmgpu_apply_S, unused (0), S.x main kernel alone (k_struct or
+ * k_sx_partial, mean of the last rbmgpu_time_sx)] in ms. */
 int rbmgpu_kernel_times(const rbmgpu_ctx* ctx, double* ms6);
 int rbmgpu_synchronize(rbmgpu_ctx* ctx);
 /* ---- learned proposal (no counterpart in the reference) */

@@ -990,6 +990,112 @@ __global__ void __launch_bounds__(256) k_sx_reduce(SxArgs<R> a) {
 // (cg_solver.cpp:301-317).  With pdot, each block also writes its partial
 // of x^H y for the CG step.
 constexpr int kColWaves = 16;
+// CG step on the finished product Ap = S p (cg_solver.cpp:151-189), by one
+// block: alpha = rs / p^H Ap from the column blocks' partials, x += alpha p,
+// r -= alpha Ap, |r|^2, the convergence test, beta, p = r + beta p, and the
+// status update.  The first kCgReg x blockDim elements stay in registers
+// and all their loads are issued before the first reduction, so the step
+// waits on global memory once; larger P walks the rest twice.
+// red: 2 x 16 doubles of LDS.
+constexpr int kCgReg = 4;
+__device__ inline void cg_update_block(const double* pdot, int nb, double* x, double* r, double* p,
+                                       const double* Ap, int P, CgStatus* st, double* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6, nt = blockDim.x;
+    auto bsum2 = [&](double& u, double& v) {
+        u = group_sum<64>(u);
+        v = group_sum<64>(v);
+        __syncthreads();  // earlier reads of red are done
+        if (lane == 0) {
+            red[w] = u;
+            red[16 + w] = v;
+        }
+        __syncthreads();
+        u = v = 0;
+        for (int q = 0; q < nw; q++) {
+            u += red[q];
+            v += red[16 + q];
+        }
+    };
+    double pv[kCgReg][2], qv[kCgReg][2], xv[kCgReg][2], rv[kCgReg][2];
+#pragma unroll
+    for (int k = 0; k < kCgReg; k++) {
+        const int i = threadIdx.x + k * nt < P ? threadIdx.x + k * nt : 0;  // P >= 1
+        pv[k][0] = p[2 * i];
+        pv[k][1] = p[2 * i + 1];
+        qv[k][0] = Ap[2 * i];
+        qv[k][1] = Ap[2 * i + 1];
+        xv[k][0] = x[2 * i];
+        xv[k][1] = x[2 * i + 1];
+        rv[k][0] = r[2 * i];
+        rv[k][1] = r[2 * i + 1];
+    }
+    const double rsold = st->rsold, rtol = st->rtol;
+    double ar = 0, ai = 0;  // p^H Ap
+    for (int b = threadIdx.x; b < nb; b += nt) {
+        ar += pdot[2 * b];
+        ai += pdot[2 * b + 1];
+    }
+    bsum2(ar, ai);
+    const double den = ar * ar + ai * ai;
+    const double alr = rsold * ar / den, ali = -rsold * ai / den;  // rsold / (ar + i ai)
+    double sr = 0, dummy = 0;
+#pragma unroll
+    for (int k = 0; k < kCgReg; k++) {
+        const int i = threadIdx.x + k * nt;
+        if (i < P) {
+            const double pr = pv[k][0], pi = pv[k][1], qr = qv[k][0], qi = qv[k][1];
+            x[2 * i] = xv[k][0] + (alr * pr - ali * pi);
+            x[2 * i + 1] = xv[k][1] + (alr * pi + ali * pr);
+            const double rr = rv[k][0] - (alr * qr - ali * qi), ri = rv[k][1] - (alr * qi + ali * qr);
+            rv[k][0] = rr;
+            rv[k][1] = ri;
+            r[2 * i] = rr;
+            r[2 * i + 1] = ri;
+            sr += rr * rr + ri * ri;
+        }
+    }
+    for (int i = threadIdx.x + kCgReg * nt; i < P; i += nt) {
+        const double pr = p[2 * i], pi = p[2 * i + 1], qr = Ap[2 * i], qi = Ap[2 * i + 1];
+        x[2 * i] += alr * pr - ali * pi;
+        x[2 * i + 1] += alr * pi + ali * pr;
+        const double rr = r[2 * i] - (alr * qr - ali * qi), ri = r[2 * i + 1] - (alr * qi + ali * qr);
+        r[2 * i] = rr;
+        r[2 * i + 1] = ri;
+        sr += rr * rr + ri * ri;
+    }
+    bsum2(sr, dummy);
+    const bool conv = sr < rtol;
+    if (!conv) {
+        const double beta = sr / rsold;
+#pragma unroll
+        for (int k = 0; k < kCgReg; k++) {
+            const int i = threadIdx.x + k * nt;
+            if (i < P) {
+                p[2 * i] = rv[k][0] + beta * pv[k][0];
+                p[2 * i + 1] = rv[k][1] + beta * pv[k][1];
+            }
+        }
+        for (int i = threadIdx.x + kCgReg * nt; i < P; i += nt) {
+            p[2 * i] = r[2 * i] + beta * p[2 * i];
+            p[2 * i + 1] = r[2 * i + 1] + beta * p[2 * i + 1];
+        }
+    }
+    if (threadIdx.x == 0) {
+        st->rs = sr;
+        if (conv) {
+            st->itn = st->it;
+            st->done = 1;
+        } else {
+            st->rsold = sr;
+            st->it += 1;
+            if (st->it >= st->max_iter) {
+                st->itn = st->max_iter;
+                st->done = 1;
+            }
+        }
+    }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     if (a.status && a.status->done) return;
@@ -999,27 +1105,47 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     const int c = blockIdx.x * kColBlock + lane;
     const int64_t per = (a.nblocks + kColWaves - 1) / kColWaves, b0 = w * per;
     const int64_t b1 = b0 + per < a.nblocks ? b0 + per : a.nblocks;
-    double zr = 0, zi = 0;
-    if (c < a.P) {
-        int64_t b = b0;
-        for (; b + 4 <= b1; b += 4) {
-            cplx<R> v0 = a.part[b * a.ld + c], v1 = a.part[(b + 1) * a.ld + c];
-            cplx<R> v2 = a.part[(b + 2) * a.ld + c], v3 = a.part[(b + 3) * a.ld + c];
-            zr += ((double)v0.re + (double)v1.re) + ((double)v2.re + (double)v3.re);
-            zi += ((double)v0.im + (double)v1.im) + ((double)v2.im + (double)v3.im);
-        }
-        for (; b < b1; b++) {
-            cplx<R> v = a.part[b * a.ld + c];
-            zr += v.re;
-            zi += v.im;
-        }
+    // every load is issued before the first use (clamped indices, masked
+    // sums): the first kCgReg x blockDim entries of mu and x for mu^T x,
+    // the column's own entries, then the wave's partial rows in batches of
+    // kColRows
+    constexpr int kColRows = 16;
+    const int cc = c < a.P ? c : a.P - 1;
+    double mu[kCgReg][4];
+#pragma unroll
+    for (int k = 0; k < kCgReg; k++) {
+        const int q = threadIdx.x + k * blockDim.x < a.P ? threadIdx.x + k * blockDim.x : 0;
+        mu[k][0] = a.mean[2 * q];
+        mu[k][1] = a.mean[2 * q + 1];
+        mu[k][2] = a.x[2 * q];
+        mu[k][3] = a.x[2 * q + 1];
     }
+    const double var_c = a.var[cc], maxvar = *a.maxvar, xr = a.x[2 * cc], xi = a.x[2 * cc + 1];
+    const double ur = a.mean[2 * cc], ui = -a.mean[2 * cc + 1];  // conj(mu_c)
+    double zr = 0, zi = 0;
+    for (int64_t bb = b0; bb < b1; bb += kColRows) {
+        cplx<R> v[kColRows];
+#pragma unroll
+        for (int k = 0; k < kColRows; k++) v[k] = a.part[(bb + k < b1 ? bb + k : b1 - 1) * a.ld + cc];
+#pragma unroll
+        for (int k = 0; k < kColRows; k++)
+            if (bb + k < b1) {
+                zr += (double)v[k].re;
+                zi += (double)v[k].im;
+            }
+    }
+    double mr = 0, mi = 0;
+#pragma unroll
+    for (int k = 0; k < kCgReg; k++)
+        if (threadIdx.x + k * blockDim.x < a.P) {
+            mr += mu[k][0] * mu[k][2] - mu[k][1] * mu[k][3];
+            mi += mu[k][0] * mu[k][3] + mu[k][1] * mu[k][2];
+        }
     zs[w][lane][0] = zr;
     zs[w][lane][1] = zi;
-    // mu^T x (all threads, P is small)
-    double mr = 0, mi = 0;
-    for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
-        double ur = a.mean[2 * p], ui = a.mean[2 * p + 1], xr = a.x[2 * p], xi = a.x[2 * p + 1];
+    // mu^T x: the rest of P (above kCgReg x blockDim)
+    for (int q = threadIdx.x + kCgReg * blockDim.x; q < a.P; q += blockDim.x) {
+        double ur = a.mean[2 * q], ui = a.mean[2 * q + 1], xr = a.x[2 * q], xi = a.x[2 * q + 1];
         mr += ur * xr - ui * xi;
         mi += ur * xi + ui * xr;
     }
@@ -1030,7 +1156,7 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
         red[1][w] = mi;
     }
     __syncthreads();
-    if (w != 0) return;
+    if (w != 0 && !a.cg_st) return;
     mr = 0;
     mi = 0;
     for (int q = 0; q < kColWaves; q++) {
@@ -1038,7 +1164,7 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
         mi += red[1][q];
     }
     double dr = 0, di = 0;
-    if (c < a.P) {
+    if (w == 0 && c < a.P) {
         zr = 0;
         zi = 0;
         for (int q = 0; q < kColWaves; q++) {
@@ -1046,9 +1172,7 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
             zi += zs[q][lane][1];
         }
         double inv = 1.0 / (double)a.S;
-        double reg = a.var[c] * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
-        double xr = a.x[2 * c], xi = a.x[2 * c + 1];
-        double ur = a.mean[2 * c], ui = -a.mean[2 * c + 1];  // conj(mu_c)
+        double reg = var_c * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + maxvar * a.r2;
         double yr = zr * inv + reg * xr - (ur * mr - ui * mi);
         double yi = zi * inv + reg * xi - (ur * mi + ui * mr);
         a.y[2 * c] = yr;
@@ -1056,13 +1180,29 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
         dr = xr * yr + xi * yi;  // conj(x) y
         di = xr * yi - xi * yr;
     }
-    if (a.pdot) {
+    if (a.pdot && w == 0) {
         dr = group_sum<64>(dr);
         di = group_sum<64>(di);
         if (lane == 0) {
             a.pdot[2 * blockIdx.x] = dr;
             a.pdot[2 * blockIdx.x + 1] = di;
         }
+    }
+    if (a.cg_st) {
+        // the last column block to finish runs the CG step on the whole
+        // product: the block's global writes are made visible device-wide
+        // (all XCDs) before it takes its ticket, and the last block acquires
+        // the others' before reading them
+        __shared__ unsigned ticket;
+        if (w == 0) __threadfence();  // wave 0 wrote y and pdot; an agent fence costs an L2 write-back
+        __syncthreads();
+        if (threadIdx.x == 0) ticket = atomicAdd(&a.cg_st->count, 1u);
+        __syncthreads();
+        if (ticket != gridDim.x - 1) return;
+        __threadfence();
+        cg_update_block(a.pdot, (int)gridDim.x, a.cg_x, a.cg_r, const_cast<double*>(a.x), a.y, a.P, a.cg_st,
+                        &red[0][0]);
+        if (threadIdx.x == 0) a.cg_st->count = 0;  // ready for the next launch
     }
 }
 

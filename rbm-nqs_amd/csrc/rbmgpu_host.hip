@@ -78,11 +78,10 @@ struct rbmgpu_ctx {
     Shape shape{0, 0};
     int block = 256;
     // timing
-    hipEvent_t ev[12] = {};
+    hipEvent_t ev[14] = {};
     double times[6] = {0, 0, 0, 0, 0, 0};
+    int64_t cg_last_itn = 0;  // iterations of the last CG solve
     std::vector<char> staging;
-    hipEvent_t sxev[34] = {};
-    hipEvent_t sxmid[16] = {};  // after the main S.x kernel of each matvec of a batch
     // ---- learned proposal
     int Hn = 0, Np = 0, kmax = 0;
     double lam = 0;
@@ -659,56 +658,7 @@ __global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double*
         st->max_iter = max_iter;
         st->rtol = rtol;
         st->done = max_iter <= 0;
-    }
-}
-
-// CG update after y = S p (cg_solver.cpp:160-189): p^H y from the column
-// blocks' partials (k_sx_colsum), then x, r, |r|^2 and p.  One block.
-__global__ void __launch_bounds__(1024) k_cg_update(const double* pdot, int nb, double* x, double* r, double* p,
-                                                    const double* Ap, int P, CgStatus* st) {
-    if (st->done) return;
-    double ar = 0, ai = 0;  // p^H Ap
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        ar += pdot[2 * b];
-        ai += pdot[2 * b + 1];
-    }
-    block_sum2(ar, ai);
-    double rsold = st->rsold;
-    double den = ar * ar + ai * ai;
-    double alr = rsold * ar / den, ali = -rsold * ai / den;  // rsold / (ar + i ai)
-    double s = 0, dummy = 0;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        double pr = p[2 * i], pi = p[2 * i + 1], qr = Ap[2 * i], qi = Ap[2 * i + 1];
-        x[2 * i] += alr * pr - ali * pi;
-        x[2 * i + 1] += alr * pi + ali * pr;
-        double rr = r[2 * i] - (alr * qr - ali * qi), ri = r[2 * i + 1] - (alr * qi + ali * qr);
-        r[2 * i] = rr;
-        r[2 * i + 1] = ri;
-        s += rr * rr + ri * ri;
-    }
-    block_sum2(s, dummy);
-    bool conv = s < st->rtol;
-    if (!conv) {
-        double beta = s / rsold;
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-            p[2 * i] = r[2 * i] + beta * p[2 * i];
-            p[2 * i + 1] = r[2 * i + 1] + beta * p[2 * i + 1];
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        st->rs = s;
-        if (conv) {
-            st->itn = st->it;
-            st->done = 1;
-        } else {
-            st->rsold = s;
-            st->it += 1;
-            if (st->it >= st->max_iter) {
-                st->itn = st->max_iter;
-                st->done = 1;
-            }
-        }
+        st->count = 0;
     }
 }
 
@@ -859,7 +809,7 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
 // block partials (the dominant kernel), optional event, reduction, finish
 template <typename R>
 int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-           hipEvent_t e0, hipEvent_t e1, double* pdot) {
+           hipEvent_t e0, hipEvent_t e1, double* pdot, bool cg) {
     rbmgpu_ctx* ctx = c;
     SxArgs<R> a = sx_args<R>(c, x, y, r1, r2, rd, st);
     if (c->structured) {
@@ -868,6 +818,12 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
         HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream, e0, e1));
     }
     a.pdot = pdot;
+    if (cg) {
+        // the CG step runs in the column sums' last block
+        a.cg_st = c->d_st;
+        a.cg_x = c->d_x;
+        a.cg_r = c->d_r;
+    }
     HIPCK(launch_sx_colsum<R>(a, c->stream));
     return 0;
 }
@@ -895,10 +851,11 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hip
 
 // e0 / e1: optional start / end events of the main kernel's dispatch;
 // pdot: optional per-column-block partials of x^H y for the CG update
+// cg: run the CG step on the product in the same launch (x = d_p, y = d_Ap)
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, double* pdot = nullptr) {
-    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1, pdot)
-                                  : run_sx<double>(c, x, y, r1, r2, rd, st, e0, e1, pdot);
+       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, double* pdot = nullptr, bool cg = false) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1, pdot, cg)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st, e0, e1, pdot, cg);
 }
 
 int compute_moments(rbmgpu_ctx* c) {
@@ -1123,8 +1080,6 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     // invalidation between the kernels they bracket); results are read
     // after a stream synchronisation
     for (auto& e : c->ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-    for (auto& e : c->sxev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-    for (auto& e : c->sxmid) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     std::vector<uint8_t> perm8(c->perm.begin(), c->perm.end());
     if (upload(ctx, &c->d_perm, perm8.data(), perm8.size())) return bad(c->err);
     std::vector<int32_t> inv((size_t)c->nsymm * c->N), vbs((size_t)c->nsymm * c->nvb);
@@ -1206,10 +1161,6 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->sxev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->sxmid)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1510,6 +1461,7 @@ int rbmgpu_time_sx(rbmgpu_ctx* c, int32_t reps, double* ms) {
     HIPCK(hipEventRecord(c->ev[9], c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     *ms = elapsed(c, 8, 9) / reps;
+    c->times[5] = *ms;
     return 0;
 }
 
@@ -1533,46 +1485,31 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
     HIPCK(hipGetLastError());
     CgStatus st{};
     int64_t launched = 0;
-    double sx_ms = 0, main_ms = 0;
-    int64_t sx_n = 0;
-    const int batch = 16;
-    while (true) {
-        int nb = (int)std::min<int64_t>(batch, mi - launched);
-        if (nb <= 0) break;
-        for (int i = 0; i < nb; i++) {
-            // sxev[2i] / sxmid[i]: timestamps of the main kernel's dispatch;
-            // sxev[2i+1]: after y = S p
-            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->sxev[2 * i], c->sxmid[i], c->d_pdot))
-                return rc;
-            HIPCK(hipEventRecord(c->sxev[2 * i + 1], c->stream));
-            hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, c->d_pdot,
-                               (int)((c->P + kColBlock - 1) / kColBlock), c->d_x, c->d_r, c->d_p, c->d_Ap,
-                               (int)c->P, c->d_st);
-            HIPCK(hipGetLastError());
-        }
+    // the loop is queued in batches with one status read-back each: the
+    // first as long as the last solve (warm-started solves of consecutive
+    // epochs converge in similar counts), then kCgBatch more at a time;
+    // launches queued past convergence return at once
+    constexpr int64_t kCgBatch = 8;
+    int64_t nb = std::max<int64_t>(kCgBatch, c->cg_last_itn + 1);
+    HIPCK(hipEventRecord(c->ev[12], c->stream));
+    while (launched < mi) {
+        nb = std::min<int64_t>(nb, mi - launched);
+        for (int64_t i = 0; i < nb; i++)
+            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, nullptr, nullptr, c->d_pdot, true)) return rc;
         HIPCK(hipMemcpyAsync(&st, c->d_st, sizeof(CgStatus), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
-        // matvecs that did work in this batch: all, or up to the converging one
-        int64_t worked = st.done ? std::min<int64_t>(nb, st.itn + 1 - launched) : nb;
-        for (int64_t i = 0; i < worked; i++) {
-            float ms = 0;
-            HIPCK(hipEventElapsedTime(&ms, c->sxev[2 * i], c->sxev[2 * i + 1]));
-            sx_ms += ms;
-            HIPCK(hipEventElapsedTime(&ms, c->sxev[2 * i], c->sxmid[i]));
-            main_ms += ms;
-            sx_n++;
-        }
         launched += nb;
+        nb = kCgBatch;
         if (st.done) break;
     }
+    HIPCK(hipEventRecord(c->ev[13], c->stream));
     HIPCK(hipMemcpyAsync(x, c->d_x, c->P * 16, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     if (itn) *itn = st.itn;
     if (rs) *rs = st.rs;
-    if (sx_n) {
-        c->times[3] = sx_ms / sx_n;
-        c->times[5] = main_ms / sx_n;
-    }
+    c->cg_last_itn = st.itn;
+    // CG iterations that did work: itn (converged) or max_iter
+    if (st.itn > 0) c->times[3] = elapsed(c, 12, 13) / (double)st.itn;
     return 0;
 }
 

@@ -290,9 +290,12 @@ class Gpu:
         return st, ob, Om, ca
 
     def kernel_times(self):
+        """ms: sample, assemble, reduce; cg_iter = one CG iteration (S.x and
+        the CG step) of the last sr_solve; sx_main = the S.x main kernel
+        alone, from the last time_sx."""
         t = np.zeros(6)
         self._chk(libs()[0].rbmgpu_kernel_times(self.c, _p(t, DP)))
-        return dict(zip(["sample", "assemble", "reduce", "sx", "cg", "sx_main"], t.tolist()))
+        return dict(zip(["sample", "assemble", "reduce", "cg_iter", "unused", "sx_main"], t.tolist()))
 
     # ---- learned proposal
     def learn_weights(self, params=None):
