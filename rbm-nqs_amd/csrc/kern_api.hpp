@@ -111,7 +111,6 @@ struct CgStatus {
     int64_t it, itn, max_iter;
     double rtol;
     int done;
-    unsigned count;  // finished column blocks of a fused S.x + CG step (0 between launches)
 };
 
 template <typename R>
@@ -129,12 +128,6 @@ struct SxArgs {
     double* y;           // complex P
     double* pdot;        // per column block: p^H y partial (2 doubles), optional
     const CgStatus* status;
-    // optional CG step after the product (x = p of the CG, y = A p): the
-    // column blocks' last one updates cg_x, cg_r, p and cg_st, counting the
-    // finished blocks in cg_st->count
-    CgStatus* cg_st;
-    double* cg_x;
-    double* cg_r;
 };
 
 // learned proposal (no counterpart in the reference); see kern_learn.hpp

@@ -990,111 +990,7 @@ __global__ void __launch_bounds__(256) k_sx_reduce(SxArgs<R> a) {
 // (cg_solver.cpp:301-317).  With pdot, each block also writes its partial
 // of x^H y for the CG step.
 constexpr int kColWaves = 16;
-// CG step on the finished product Ap = S p (cg_solver.cpp:151-189), by one
-// block: alpha = rs / p^H Ap from the column blocks' partials, x += alpha p,
-// r -= alpha Ap, |r|^2, the convergence test, beta, p = r + beta p, and the
-// status update.  The first kCgReg x blockDim elements stay in registers
-// and all their loads are issued before the first reduction, so the step
-// waits on global memory once; larger P walks the rest twice.
-// red: 2 x 16 doubles of LDS.
-constexpr int kCgReg = 4;
-__device__ inline void cg_update_block(const double* pdot, int nb, double* x, double* r, double* p,
-                                       const double* Ap, int P, CgStatus* st, double* red) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6, nt = blockDim.x;
-    auto bsum2 = [&](double& u, double& v) {
-        u = group_sum<64>(u);
-        v = group_sum<64>(v);
-        __syncthreads();  // earlier reads of red are done
-        if (lane == 0) {
-            red[w] = u;
-            red[16 + w] = v;
-        }
-        __syncthreads();
-        u = v = 0;
-        for (int q = 0; q < nw; q++) {
-            u += red[q];
-            v += red[16 + q];
-        }
-    };
-    double pv[kCgReg][2], qv[kCgReg][2], xv[kCgReg][2], rv[kCgReg][2];
-#pragma unroll
-    for (int k = 0; k < kCgReg; k++) {
-        const int i = threadIdx.x + k * nt < P ? threadIdx.x + k * nt : 0;  // P >= 1
-        pv[k][0] = p[2 * i];
-        pv[k][1] = p[2 * i + 1];
-        qv[k][0] = Ap[2 * i];
-        qv[k][1] = Ap[2 * i + 1];
-        xv[k][0] = x[2 * i];
-        xv[k][1] = x[2 * i + 1];
-        rv[k][0] = r[2 * i];
-        rv[k][1] = r[2 * i + 1];
-    }
-    const double rsold = st->rsold, rtol = st->rtol;
-    double ar = 0, ai = 0;  // p^H Ap
-    for (int b = threadIdx.x; b < nb; b += nt) {
-        ar += pdot[2 * b];
-        ai += pdot[2 * b + 1];
-    }
-    bsum2(ar, ai);
-    const double den = ar * ar + ai * ai;
-    const double alr = rsold * ar / den, ali = -rsold * ai / den;  // rsold / (ar + i ai)
-    double sr = 0, dummy = 0;
-#pragma unroll
-    for (int k = 0; k < kCgReg; k++) {
-        const int i = threadIdx.x + k * nt;
-        if (i < P) {
-            const double pr = pv[k][0], pi = pv[k][1], qr = qv[k][0], qi = qv[k][1];
-            x[2 * i] = xv[k][0] + (alr * pr - ali * pi);
-            x[2 * i + 1] = xv[k][1] + (alr * pi + ali * pr);
-            const double rr = rv[k][0] - (alr * qr - ali * qi), ri = rv[k][1] - (alr * qi + ali * qr);
-            rv[k][0] = rr;
-            rv[k][1] = ri;
-            r[2 * i] = rr;
-            r[2 * i + 1] = ri;
-            sr += rr * rr + ri * ri;
-        }
-    }
-    for (int i = threadIdx.x + kCgReg * nt; i < P; i += nt) {
-        const double pr = p[2 * i], pi = p[2 * i + 1], qr = Ap[2 * i], qi = Ap[2 * i + 1];
-        x[2 * i] += alr * pr - ali * pi;
-        x[2 * i + 1] += alr * pi + ali * pr;
-        const double rr = r[2 * i] - (alr * qr - ali * qi), ri = r[2 * i + 1] - (alr * qi + ali * qr);
-        r[2 * i] = rr;
-        r[2 * i + 1] = ri;
-        sr += rr * rr + ri * ri;
-    }
-    bsum2(sr, dummy);
-    const bool conv = sr < rtol;
-    if (!conv) {
-        const double beta = sr / rsold;
-#pragma unroll
-        for (int k = 0; k < kCgReg; k++) {
-            const int i = threadIdx.x + k * nt;
-            if (i < P) {
-                p[2 * i] = rv[k][0] + beta * pv[k][0];
-                p[2 * i + 1] = rv[k][1] + beta * pv[k][1];
-            }
-        }
-        for (int i = threadIdx.x + kCgReg * nt; i < P; i += nt) {
-            p[2 * i] = r[2 * i] + beta * p[2 * i];
-            p[2 * i + 1] = r[2 * i + 1] + beta * p[2 * i + 1];
-        }
-    }
-    if (threadIdx.x == 0) {
-        st->rs = sr;
-        if (conv) {
-            st->itn = st->it;
-            st->done = 1;
-        } else {
-            st->rsold = sr;
-            st->it += 1;
-            if (st->it >= st->max_iter) {
-                st->itn = st->max_iter;
-                st->done = 1;
-            }
-        }
-    }
-}
+constexpr int kColMu = 4;  // entries of mu and x per thread held in registers
 
 template <typename R>
 __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
@@ -1106,14 +1002,14 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     const int64_t per = (a.nblocks + kColWaves - 1) / kColWaves, b0 = w * per;
     const int64_t b1 = b0 + per < a.nblocks ? b0 + per : a.nblocks;
     // every load is issued before the first use (clamped indices, masked
-    // sums): the first kCgReg x blockDim entries of mu and x for mu^T x,
+    // sums): the first kColMu x blockDim entries of mu and x for mu^T x,
     // the column's own entries, then the wave's partial rows in batches of
     // kColRows
     constexpr int kColRows = 16;
     const int cc = c < a.P ? c : a.P - 1;
-    double mu[kCgReg][4];
+    double mu[kColMu][4];
 #pragma unroll
-    for (int k = 0; k < kCgReg; k++) {
+    for (int k = 0; k < kColMu; k++) {
         const int q = threadIdx.x + k * blockDim.x < a.P ? threadIdx.x + k * blockDim.x : 0;
         mu[k][0] = a.mean[2 * q];
         mu[k][1] = a.mean[2 * q + 1];
@@ -1136,15 +1032,15 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     }
     double mr = 0, mi = 0;
 #pragma unroll
-    for (int k = 0; k < kCgReg; k++)
+    for (int k = 0; k < kColMu; k++)
         if (threadIdx.x + k * blockDim.x < a.P) {
             mr += mu[k][0] * mu[k][2] - mu[k][1] * mu[k][3];
             mi += mu[k][0] * mu[k][3] + mu[k][1] * mu[k][2];
         }
     zs[w][lane][0] = zr;
     zs[w][lane][1] = zi;
-    // mu^T x: the rest of P (above kCgReg x blockDim)
-    for (int q = threadIdx.x + kCgReg * blockDim.x; q < a.P; q += blockDim.x) {
+    // mu^T x: the rest of P (above kColMu x blockDim)
+    for (int q = threadIdx.x + kColMu * blockDim.x; q < a.P; q += blockDim.x) {
         double ur = a.mean[2 * q], ui = a.mean[2 * q + 1], xr = a.x[2 * q], xi = a.x[2 * q + 1];
         mr += ur * xr - ui * xi;
         mi += ur * xi + ui * xr;
@@ -1156,7 +1052,7 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
         red[1][w] = mi;
     }
     __syncthreads();
-    if (w != 0 && !a.cg_st) return;
+    if (w != 0) return;
     mr = 0;
     mi = 0;
     for (int q = 0; q < kColWaves; q++) {
@@ -1164,7 +1060,7 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
         mi += red[1][q];
     }
     double dr = 0, di = 0;
-    if (w == 0 && c < a.P) {
+    if (c < a.P) {
         zr = 0;
         zi = 0;
         for (int q = 0; q < kColWaves; q++) {
@@ -1180,29 +1076,13 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
         dr = xr * yr + xi * yi;  // conj(x) y
         di = xr * yi - xi * yr;
     }
-    if (a.pdot && w == 0) {
+    if (a.pdot) {
         dr = group_sum<64>(dr);
         di = group_sum<64>(di);
         if (lane == 0) {
             a.pdot[2 * blockIdx.x] = dr;
             a.pdot[2 * blockIdx.x + 1] = di;
         }
-    }
-    if (a.cg_st) {
-        // the last column block to finish runs the CG step on the whole
-        // product: the block's global writes are made visible device-wide
-        // (all XCDs) before it takes its ticket, and the last block acquires
-        // the others' before reading them
-        __shared__ unsigned ticket;
-        if (w == 0) __threadfence();  // wave 0 wrote y and pdot; an agent fence costs an L2 write-back
-        __syncthreads();
-        if (threadIdx.x == 0) ticket = atomicAdd(&a.cg_st->count, 1u);
-        __syncthreads();
-        if (ticket != gridDim.x - 1) return;
-        __threadfence();
-        cg_update_block(a.pdot, (int)gridDim.x, a.cg_x, a.cg_r, const_cast<double*>(a.x), a.y, a.P, a.cg_st,
-                        &red[0][0]);
-        if (threadIdx.x == 0) a.cg_st->count = 0;  // ready for the next launch
     }
 }
 

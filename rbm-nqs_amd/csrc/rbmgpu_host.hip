@@ -60,7 +60,8 @@ struct rbmgpu_ctx {
     int64_t ld_s = 0;  // structured partial row: [P | sum |t|^2 (M)], even
     // moments
     double *d_part = nullptr, *d_omean = nullptr, *d_ovar = nullptr, *d_eo = nullptr,
-           *d_maxvar = nullptr, *d_emom = nullptr;
+           *d_maxvar = nullptr, *d_emom = nullptr, *d_epart = nullptr;
+    int64_t epart_cap = 0;  // bytes of d_epart
     int64_t part_cap = 0;
     bool moments_valid = false;
     // CG / S.x
@@ -532,74 +533,6 @@ float elapsed(rbmgpu_ctx* c, int a, int b) {
 }
 
 // ---- reductions in FP64 (aggregator.cpp:84-239, one rank, unit weights)
-__global__ void __launch_bounds__(1024) k_e_moments(const double* obs, int64_t S, int n_groups, int64_t n_bins,
-                                                    double* out) {
-    __shared__ double ra[1024], rb[1024];
-    auto reduce = [&](double& x, double& y) {
-        ra[threadIdx.x] = x;
-        rb[threadIdx.x] = y;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                ra[threadIdx.x] += ra[threadIdx.x + s];
-                rb[threadIdx.x] += rb[threadIdx.x + s];
-            }
-            __syncthreads();
-        }
-        x = ra[0];
-        y = rb[0];
-        __syncthreads();
-    };
-    int G = 2 * n_groups;
-    // means of every group
-    for (int g = 0; g < n_groups; g++) {
-        double sr = 0, si = 0;
-        for (int64_t k = threadIdx.x; k < S; k += blockDim.x) {
-            sr += obs[k * G + 2 * g];
-            si += obs[k * G + 2 * g + 1];
-        }
-        reduce(sr, si);
-        if (threadIdx.x == 0) {
-            out[16 + 2 * g] = sr / (double)S;
-            out[16 + 2 * g + 1] = si / (double)S;
-        }
-    }
-    __syncthreads();
-    double mr = out[16], mi = out[17];
-    double v = 0, dummy = 0;
-    for (int64_t k = threadIdx.x; k < S; k += blockDim.x) {
-        double dr = obs[k * G] - mr, di = obs[k * G + 1] - mi;
-        v += dr * dr + di * di;
-    }
-    reduce(v, dummy);
-    double vb = 0;
-    int64_t bs = n_bins > 1 ? S / n_bins : 0;
-    if (n_bins > 1) {
-        for (int64_t b = 0; b < n_bins; b++) {
-            double br = 0, bi = 0;
-            for (int64_t k = b * bs + threadIdx.x; k < (b + 1) * bs; k += blockDim.x) {
-                br += obs[k * G];
-                bi += obs[k * G + 1];
-            }
-            reduce(br, bi);
-            double dr = br / bs - mr, di = bi / bs - mi;
-            vb += bs * (dr * dr + di * di);
-        }
-    }
-    if (threadIdx.x == 0) {
-        double denom = (double)S - 1.0;  // ptotal - wsum2 / ptotal
-        double tau = 0;
-        if (n_bins > 1 && !(v == 0 && vb == 0)) tau = 0.5 * (fmax(bs * vb / v, 1.0) - 1);
-        double var = fmax(v / denom, 0.0), varb = fmax(vb / denom, 0.0);
-        out[0] = mr;
-        out[1] = mi;
-        out[2] = var;
-        out[3] = varb;
-        out[4] = tau;
-        out[5] = n_bins > 1 ? sqrt(varb / (double)n_bins) : sqrt(var / (double)S);
-    }
-}
-
 __global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb, int64_t S, int P,
                                                  double* mean, double* var, double* eo,
                                                  unsigned long long* maxvar) {
@@ -621,21 +554,127 @@ __global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb,
 }
 
 // ---- CG on the device, FP64 vectors (cg_solver.cpp:119-189)
+// block-wide sums of x and y, returned to every thread (blockDim a multiple
+// of 64, at most 1024): wave shuffles, then one LDS pass over the waves
 __device__ void block_sum2(double& x, double& y) {
-    __shared__ double ra[1024], rb[1024];
-    ra[threadIdx.x] = x;
-    rb[threadIdx.x] = y;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            ra[threadIdx.x] += ra[threadIdx.x + s];
-            rb[threadIdx.x] += rb[threadIdx.x + s];
-        }
-        __syncthreads();
+    __shared__ double ra[16], rb[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    x = group_sum<64>(x);
+    y = group_sum<64>(y);
+    if (lane == 0) {
+        ra[w] = x;
+        rb[w] = y;
     }
-    x = ra[0];
-    y = rb[0];
     __syncthreads();
+    x = y = 0;
+    for (int q = 0; q < nw; q++) {
+        x += ra[q];
+        y += rb[q];
+    }
+    __syncthreads();  // ra / rb may be written again by the next call
+}
+
+// Energy moments (aggregator.cpp:84-131 with binning, unit weights) in two
+// launches.  k_e_part: one block per range of `chunk` samples (a range never
+// straddles a bin): the range's sum of every group, and for the energy
+// (group 0) the sum of squared deviations from the range's own mean.
+// part row: [M2, n, (sum re, sum im) x n_groups].
+__global__ void __launch_bounds__(256) k_e_part(const double* obs, int64_t S, int n_groups, int64_t chunk,
+                                                double* part) {
+    const int G = 2 * n_groups;
+    const int64_t k0 = (int64_t)blockIdx.x * chunk, k1 = k0 + chunk < S ? k0 + chunk : S;
+    double* out = part + (int64_t)blockIdx.x * (2 + G);
+    double mr = 0, mi = 0;
+    for (int g = 0; g < n_groups; g++) {
+        double sr = 0, si = 0;
+        for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            sr += obs[k * G + 2 * g];
+            si += obs[k * G + 2 * g + 1];
+        }
+        block_sum2(sr, si);
+        if (g == 0) {
+            mr = sr / (double)(k1 - k0);
+            mi = si / (double)(k1 - k0);
+        }
+        if (threadIdx.x == 0) {
+            out[2 + 2 * g] = sr;
+            out[3 + 2 * g] = si;
+        }
+    }
+    double v = 0, dummy = 0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        const double dr = obs[k * G] - mr, di = obs[k * G + 1] - mi;
+        v += dr * dr + di * di;
+    }
+    block_sum2(v, dummy);
+    if (threadIdx.x == 0) {
+        out[0] = v;
+        out[1] = (double)(k1 - k0);
+    }
+}
+
+// k_e_final, one block: the means from the range sums, the energy's sum of
+// squared deviations by the pairwise merge M2 = sum_b (M2_b + n_b |m_b - m|^2)
+// (Chan, Golub and LeVeque; the reference merges its ranks' partial results
+// the same way, aggregator.cpp:101-116), tau as aggregator.cpp:120-131, and
+// the binned variance from the bin sums
+// (per_bin ranges per bin).  out: [mean re, mean im, var, var_binned, tau,
+// std, ..., group means from 16].
+__global__ void __launch_bounds__(1024) k_e_final(const double* part, int64_t nblk, int64_t per_bin, int64_t S,
+                                                  int n_groups, int64_t n_bins, double* out) {
+    const int PS = 2 + 2 * n_groups;
+    double mr = 0, mi = 0;
+    for (int g = 0; g < n_groups; g++) {
+        double sr = 0, si = 0;
+        for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
+            sr += part[b * PS + 2 + 2 * g];
+            si += part[b * PS + 3 + 2 * g];
+        }
+        block_sum2(sr, si);
+        sr /= (double)S;
+        si /= (double)S;
+        if (g == 0) {
+            mr = sr;
+            mi = si;
+        }
+        if (threadIdx.x == 0) {
+            out[16 + 2 * g] = sr;
+            out[16 + 2 * g + 1] = si;
+        }
+    }
+    double v = 0, dummy = 0;
+    for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double n = part[b * PS + 1];
+        const double dr = part[b * PS + 2] / n - mr, di = part[b * PS + 3] / n - mi;
+        v += part[b * PS] + n * (dr * dr + di * di);
+    }
+    block_sum2(v, dummy);
+    double vb = 0;
+    const int64_t bs = n_bins > 1 ? S / n_bins : 0;
+    if (n_bins > 1) {
+        for (int64_t b = threadIdx.x; b < n_bins; b += blockDim.x) {
+            double br = 0, bi = 0;
+            for (int64_t j = 0; j < per_bin; j++) {
+                br += part[(b * per_bin + j) * PS + 2];
+                bi += part[(b * per_bin + j) * PS + 3];
+            }
+            const double dr = br / bs - mr, di = bi / bs - mi;
+            vb += bs * (dr * dr + di * di);
+        }
+        block_sum2(vb, dummy);
+    }
+    if (threadIdx.x == 0) {
+        double denom = (double)S - 1.0;  // ptotal - wsum2 / ptotal
+        double tau = 0;
+        if (n_bins > 1 && !(v == 0 && vb == 0)) tau = 0.5 * (fmax(bs * vb / v, 1.0) - 1);
+        double var = fmax(v / denom, 0.0), varb = fmax(vb / denom, 0.0);
+        out[0] = mr;
+        out[1] = mi;
+        out[2] = var;
+        out[3] = varb;
+        out[4] = tau;
+        out[5] = n_bins > 1 ? sqrt(varb / (double)n_bins) : sqrt(var / (double)S);
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double* Ax, double* r, double* p,
@@ -658,7 +697,97 @@ __global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double*
         st->max_iter = max_iter;
         st->rtol = rtol;
         st->done = max_iter <= 0;
-        st->count = 0;
+    }
+}
+
+// CG update after y = S p (cg_solver.cpp:160-189), one block: alpha =
+// rs / p^H Ap from the column blocks' partials (k_sx_colsum), x += alpha p,
+// r -= alpha Ap, |r|^2, the convergence test, beta, p = r + beta p, and the
+// status update.  The first kCgReg x blockDim elements stay in registers and
+// all their loads are issued before the first reduction, so the step waits
+// on global memory once; larger P walks the rest twice.
+constexpr int kCgReg = 4;
+__global__ void __launch_bounds__(1024) k_cg_update(const double* pdot, int nb, double* x, double* r, double* p,
+                                                    const double* Ap, int P, CgStatus* st) {
+    if (st->done) return;
+    const int nt = blockDim.x;
+    double pv[kCgReg][2], qv[kCgReg][2], xv[kCgReg][2], rv[kCgReg][2];
+#pragma unroll
+    for (int k = 0; k < kCgReg; k++) {
+        const int i = threadIdx.x + k * nt < P ? threadIdx.x + k * nt : 0;  // P >= 1
+        pv[k][0] = p[2 * i];
+        pv[k][1] = p[2 * i + 1];
+        qv[k][0] = Ap[2 * i];
+        qv[k][1] = Ap[2 * i + 1];
+        xv[k][0] = x[2 * i];
+        xv[k][1] = x[2 * i + 1];
+        rv[k][0] = r[2 * i];
+        rv[k][1] = r[2 * i + 1];
+    }
+    const double rsold = st->rsold, rtol = st->rtol;
+    double ar = 0, ai = 0;  // p^H Ap
+    for (int b = threadIdx.x; b < nb; b += nt) {
+        ar += pdot[2 * b];
+        ai += pdot[2 * b + 1];
+    }
+    block_sum2(ar, ai);
+    const double den = ar * ar + ai * ai;
+    const double alr = rsold * ar / den, ali = -rsold * ai / den;  // rsold / (ar + i ai)
+    double sr = 0, dummy = 0;
+#pragma unroll
+    for (int k = 0; k < kCgReg; k++) {
+        const int i = threadIdx.x + k * nt;
+        if (i < P) {
+            const double pr = pv[k][0], pi = pv[k][1], qr = qv[k][0], qi = qv[k][1];
+            x[2 * i] = xv[k][0] + (alr * pr - ali * pi);
+            x[2 * i + 1] = xv[k][1] + (alr * pi + ali * pr);
+            const double rr = rv[k][0] - (alr * qr - ali * qi), ri = rv[k][1] - (alr * qi + ali * qr);
+            rv[k][0] = rr;
+            rv[k][1] = ri;
+            r[2 * i] = rr;
+            r[2 * i + 1] = ri;
+            sr += rr * rr + ri * ri;
+        }
+    }
+    for (int i = threadIdx.x + kCgReg * nt; i < P; i += nt) {
+        const double pr = p[2 * i], pi = p[2 * i + 1], qr = Ap[2 * i], qi = Ap[2 * i + 1];
+        x[2 * i] += alr * pr - ali * pi;
+        x[2 * i + 1] += alr * pi + ali * pr;
+        const double rr = r[2 * i] - (alr * qr - ali * qi), ri = r[2 * i + 1] - (alr * qi + ali * qr);
+        r[2 * i] = rr;
+        r[2 * i + 1] = ri;
+        sr += rr * rr + ri * ri;
+    }
+    block_sum2(sr, dummy);
+    const bool conv = sr < rtol;
+    if (!conv) {
+        const double beta = sr / rsold;
+#pragma unroll
+        for (int k = 0; k < kCgReg; k++) {
+            const int i = threadIdx.x + k * nt;
+            if (i < P) {
+                p[2 * i] = rv[k][0] + beta * pv[k][0];
+                p[2 * i + 1] = rv[k][1] + beta * pv[k][1];
+            }
+        }
+        for (int i = threadIdx.x + kCgReg * nt; i < P; i += nt) {
+            p[2 * i] = r[2 * i] + beta * p[2 * i];
+            p[2 * i + 1] = r[2 * i + 1] + beta * p[2 * i + 1];
+        }
+    }
+    if (threadIdx.x == 0) {
+        st->rs = sr;
+        if (conv) {
+            st->itn = st->it;
+            st->done = 1;
+        } else {
+            st->rsold = sr;
+            st->it += 1;
+            if (st->it >= st->max_iter) {
+                st->itn = st->max_iter;
+                st->done = 1;
+            }
+        }
     }
 }
 
@@ -818,13 +947,13 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
         HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream, e0, e1));
     }
     a.pdot = pdot;
-    if (cg) {
-        // the CG step runs in the column sums' last block
-        a.cg_st = c->d_st;
-        a.cg_x = c->d_x;
-        a.cg_r = c->d_r;
-    }
     HIPCK(launch_sx_colsum<R>(a, c->stream));
+    if (cg) {
+        hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, pdot,
+                           (int)((c->P + kColBlock - 1) / kColBlock), c->d_x, c->d_r, c->d_p, c->d_Ap, (int)c->P,
+                           c->d_st);
+        HIPCK(hipGetLastError());
+    }
     return 0;
 }
 
@@ -851,7 +980,7 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hip
 
 // e0 / e1: optional start / end events of the main kernel's dispatch;
 // pdot: optional per-column-block partials of x^H y for the CG update
-// cg: run the CG step on the product in the same launch (x = d_p, y = d_Ap)
+// cg: the CG step on the product follows (x = d_p, y = d_Ap, pdot given)
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, double* pdot = nullptr, bool cg = false) {
     return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1, pdot, cg)
@@ -868,11 +997,33 @@ int compute_moments(rbmgpu_ctx* c) {
         if (int rc = dalloc(ctx, &c->d_eo, c->P * 2 * sizeof(double))) return rc;
         if (int rc = dalloc(ctx, &c->d_maxvar, sizeof(double))) return rc;
     }
+    // bins: the largest count <= 50 that divides S (abstract_optimizer.cpp:188-190);
+    // ranges of about 1024 samples that tile every bin
     int64_t nb = 50;
     while (c->S % nb != 0) nb--;
+    const int64_t unit = nb > 1 ? c->S / nb : c->S;
+    int64_t chunk = 1024;
+    if (nb > 1) {
+        chunk = unit;
+        for (int64_t d = std::min<int64_t>(unit, 1024); d >= 64; d--)
+            if (unit % d == 0) {
+                chunk = d;
+                break;
+            }
+    }
+    const int64_t per_bin = nb > 1 ? unit / chunk : 0;
+    const int64_t nblk = nb > 1 ? nb * per_bin : (c->S + chunk - 1) / chunk;
+    const int64_t pneed = nblk * (2 + 2 * c->n_groups) * (int64_t)sizeof(double);
+    if (pneed > c->epart_cap) {
+        if (int rc = dalloc(ctx, &c->d_epart, pneed)) return rc;
+        c->epart_cap = pneed;
+    }
     HIPCK(hipEventRecord(c->ev[4], c->stream));
-    hipLaunchKernelGGL(k_e_moments, dim3(1), dim3(1024), 0, c->stream, c->d_obs, c->S, c->n_groups, nb,
-                       c->d_emom);
+    hipLaunchKernelGGL(k_e_part, dim3((unsigned)nblk), dim3(256), 0, c->stream, c->d_obs, c->S, c->n_groups, chunk,
+                       c->d_epart);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_e_final, dim3(1), dim3(1024), 0, c->stream, c->d_epart, nblk, per_bin, c->S, c->n_groups,
+                       nb, c->d_emom);
     HIPCK(hipGetLastError());
     if (c->structured) {
         if (int rc = struct_moments(c)) return rc;
@@ -1152,7 +1303,7 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_invperm, c->d_vbsite, c->d_pb,     c->d_tk,    c->d_tgroup, c->d_rowbase, c->d_rowptr,
                     c->d_tsites, c->d_mask,   c->d_val,    c->d_states, c->d_obs,   c->d_T,      c->d_O,
                     c->d_acc,    c->d_prop,   c->d_part,   c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
-                    c->d_emom,   c->d_x,      c->d_b,      c->d_r,     c->d_p,      c->d_Ap,     c->d_st,
+                    c->d_emom,   c->d_epart,  c->d_x,      c->d_b,      c->d_r,     c->d_p,      c->d_Ap,     c->d_st,
                     c->d_sxpart, c->d_sxpart2, c->d_evst,   c->d_evout,  c->d_evobs, c->d_bs,
                     c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,
                     c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,

@@ -103,6 +103,24 @@ REDUCE_CASES = [("C1", 64, False), ("C1", 32, False), ("C1", 32, True), ("C2", 6
                 ("C2", 32, False), ("C2", 32, True), ("C3", 64, False), ("C3", 32, False)]
 
 
+# energy moments at sample counts that give every range layout of the
+# two-pass reduction: 50 bins of 120 (one range per bin), 32 bins of 2048
+# (two ranges per bin), 2 bins of 1031 (a bin with no divisor in 64..1024:
+# one range), a prime count (no binning, ragged last range)
+@pytest.mark.gpu
+@pytest.mark.parametrize("S", [6000, 65536, 2062, 3001])
+def test_energy_moments_sample_counts(S):
+    m, o, g, rng = _setup(SPECS["C1"], 64, std=0.2)
+    states = po.random_states(rng, S, m.N)
+    g.load_samples(states)
+    ref = o.aggregate(o.eloc(states))
+    got = g.reduce()
+    tol = 1e-10
+    assert abs(got["mean"] - ref["mean"]) <= tol * (1 + abs(ref["mean"]))
+    for k in ("var", "var_binned", "tau", "std"):
+        assert abs(got[k] - ref[k]) <= 10 * tol * (1 + abs(ref[k])), k
+
+
 # every tile shape of the structured S.x: (model, n_cells, alpha) -> (CT, RT)
 # = hidden / spin tiles of 16; (8, x) loads its chunks into registers, the
 # others through the LDS chunk buffers
