@@ -990,8 +990,6 @@ __global__ void __launch_bounds__(256) k_sx_reduce(SxArgs<R> a) {
 // (cg_solver.cpp:301-317).  With pdot, each block also writes its partial
 // of x^H y for the CG step.
 constexpr int kColWaves = 16;
-constexpr int kColMu = 4;  // entries of mu and x per thread held in registers
-
 template <typename R>
 __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     if (a.status && a.status->done) return;
@@ -1001,47 +999,27 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
     const int c = blockIdx.x * kColBlock + lane;
     const int64_t per = (a.nblocks + kColWaves - 1) / kColWaves, b0 = w * per;
     const int64_t b1 = b0 + per < a.nblocks ? b0 + per : a.nblocks;
-    // every load is issued before the first use (clamped indices, masked
-    // sums): the first kColMu x blockDim entries of mu and x for mu^T x,
-    // the column's own entries, then the wave's partial rows in batches of
-    // kColRows
-    constexpr int kColRows = 16;
-    const int cc = c < a.P ? c : a.P - 1;
-    double mu[kColMu][4];
-#pragma unroll
-    for (int k = 0; k < kColMu; k++) {
-        const int q = threadIdx.x + k * blockDim.x < a.P ? threadIdx.x + k * blockDim.x : 0;
-        mu[k][0] = a.mean[2 * q];
-        mu[k][1] = a.mean[2 * q + 1];
-        mu[k][2] = a.x[2 * q];
-        mu[k][3] = a.x[2 * q + 1];
-    }
-    const double var_c = a.var[cc], maxvar = *a.maxvar, xr = a.x[2 * cc], xi = a.x[2 * cc + 1];
-    const double ur = a.mean[2 * cc], ui = -a.mean[2 * cc + 1];  // conj(mu_c)
     double zr = 0, zi = 0;
-    for (int64_t bb = b0; bb < b1; bb += kColRows) {
-        cplx<R> v[kColRows];
-#pragma unroll
-        for (int k = 0; k < kColRows; k++) v[k] = a.part[(bb + k < b1 ? bb + k : b1 - 1) * a.ld + cc];
-#pragma unroll
-        for (int k = 0; k < kColRows; k++)
-            if (bb + k < b1) {
-                zr += (double)v[k].re;
-                zi += (double)v[k].im;
-            }
-    }
-    double mr = 0, mi = 0;
-#pragma unroll
-    for (int k = 0; k < kColMu; k++)
-        if (threadIdx.x + k * blockDim.x < a.P) {
-            mr += mu[k][0] * mu[k][2] - mu[k][1] * mu[k][3];
-            mi += mu[k][0] * mu[k][3] + mu[k][1] * mu[k][2];
+    if (c < a.P) {
+        int64_t b = b0;
+        for (; b + 4 <= b1; b += 4) {
+            cplx<R> v0 = a.part[b * a.ld + c], v1 = a.part[(b + 1) * a.ld + c];
+            cplx<R> v2 = a.part[(b + 2) * a.ld + c], v3 = a.part[(b + 3) * a.ld + c];
+            zr += ((double)v0.re + (double)v1.re) + ((double)v2.re + (double)v3.re);
+            zi += ((double)v0.im + (double)v1.im) + ((double)v2.im + (double)v3.im);
         }
+        for (; b < b1; b++) {
+            cplx<R> v = a.part[b * a.ld + c];
+            zr += v.re;
+            zi += v.im;
+        }
+    }
     zs[w][lane][0] = zr;
     zs[w][lane][1] = zi;
-    // mu^T x: the rest of P (above kColMu x blockDim)
-    for (int q = threadIdx.x + kColMu * blockDim.x; q < a.P; q += blockDim.x) {
-        double ur = a.mean[2 * q], ui = a.mean[2 * q + 1], xr = a.x[2 * q], xi = a.x[2 * q + 1];
+    // mu^T x (all threads, P is small)
+    double mr = 0, mi = 0;
+    for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
+        double ur = a.mean[2 * p], ui = a.mean[2 * p + 1], xr = a.x[2 * p], xi = a.x[2 * p + 1];
         mr += ur * xr - ui * xi;
         mi += ur * xi + ui * xr;
     }
@@ -1068,7 +1046,9 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
             zi += zs[q][lane][1];
         }
         double inv = 1.0 / (double)a.S;
-        double reg = var_c * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + maxvar * a.r2;
+        double reg = a.var[c] * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
+        double xr = a.x[2 * c], xi = a.x[2 * c + 1];
+        double ur = a.mean[2 * c], ui = -a.mean[2 * c + 1];  // conj(mu_c)
         double yr = zr * inv + reg * xr - (ur * mr - ui * mi);
         double yi = zi * inv + reg * xi - (ur * mi + ui * mr);
         a.y[2 * c] = yr;
