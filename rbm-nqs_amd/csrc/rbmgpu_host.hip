@@ -935,16 +935,16 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
     return a;
 }
 
-// block partials (the dominant kernel), optional event, reduction, finish
+// block partials (the dominant kernel), reduction, finish, optional CG step
 template <typename R>
 int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-           hipEvent_t e0, hipEvent_t e1, double* pdot, bool cg) {
+           double* pdot, bool cg) {
     rbmgpu_ctx* ctx = c;
     SxArgs<R> a = sx_args<R>(c, x, y, r1, r2, rd, st);
     if (c->structured) {
-        if (int rc = run_struct(c, 0, x, st, e0, e1)) return rc;
+        if (int rc = run_struct(c, 0, x, st)) return rc;
     } else {
-        HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream, e0, e1));
+        HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream));
     }
     a.pdot = pdot;
     HIPCK(launch_sx_colsum<R>(a, c->stream));
@@ -978,13 +978,12 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hip
     return 0;
 }
 
-// e0 / e1: optional start / end events of the main kernel's dispatch;
 // pdot: optional per-column-block partials of x^H y for the CG update
 // cg: the CG step on the product follows (x = d_p, y = d_Ap, pdot given)
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, double* pdot = nullptr, bool cg = false) {
-    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, e0, e1, pdot, cg)
-                                  : run_sx<double>(c, x, y, r1, r2, rd, st, e0, e1, pdot, cg);
+       double* pdot = nullptr, bool cg = false) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, pdot, cg)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st, pdot, cg);
 }
 
 int compute_moments(rbmgpu_ctx* c) {
@@ -1646,7 +1645,7 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
     while (launched < mi) {
         nb = std::min<int64_t>(nb, mi - launched);
         for (int64_t i = 0; i < nb; i++)
-            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, nullptr, nullptr, c->d_pdot, true)) return rc;
+            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->d_pdot, true)) return rc;
         HIPCK(hipMemcpyAsync(&st, c->d_st, sizeof(CgStatus), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         launched += nb;
