@@ -103,6 +103,43 @@ REDUCE_CASES = [("C1", 64, False), ("C1", 32, False), ("C1", 32, True), ("C2", 6
                 ("C2", 32, False), ("C2", 32, True), ("C3", 64, False), ("C3", 32, False)]
 
 
+# the CG loop queues its matvecs in batches, the first as long as the last
+# solve: a run of solves whose iteration counts rise and fall crosses batch
+# ends in every way (fixed counts), and a solve that converges mid-batch
+# stops where the oracle's does (cg_solver.cpp:151-189)
+@pytest.mark.gpu
+def test_cg_batches_and_early_stop():
+    m, o, g, rng = _setup(SPECS["C1"], 64, std=0.2)
+    S = 200
+    states = po.random_states(rng, S, m.N)
+    g.load_samples(states)
+    e = o.eloc(states)
+    O = o.derivative(states)
+    ref = o.aggregate(e, O)
+    g.reduce()
+    r1, r2, rd = 0.9, 9e-4, 1e-2
+    F = ref["eo"] - ref["mean"] * ref["o_mean"].conj()
+    x0 = 0.01 * (rng.normal(size=m.P) + 1j * rng.normal(size=m.P))
+    args = (O, ref["o_mean"], ref["o_var"], r1, r2, rd, m.n_neural, F, x0)
+    for mi in (2, 10, 3, 9, 1):
+        xg, itg, rsg = g.sr_solve(F, x0, r1, r2, rd, max_iter=mi, rtol=1e-300)
+        xo, ito, rso = po.cg(*args, mi, 1e-300)
+        assert itg == ito == mi, mi
+        assert np.max(np.abs(xg - xo)) <= 1e-9 * np.abs(xo).max(), mi
+    # converge in the fifth step: rtol between the oracle's |r|^2 after 4
+    # and 5 steps; the count is the loop index at the break, 4
+    # (cg_solver.cpp:149-175)
+    _, _, rs4 = po.cg(*args, 4, 1e-300)
+    _, _, rs5 = po.cg(*args, 5, 1e-300)
+    assert rs5 < rs4
+    rtol = (rs4 * rs5) ** 0.5
+    xo, ito, rso = po.cg(*args, 50, rtol)
+    xg, itg, rsg = g.sr_solve(F, x0, r1, r2, rd, max_iter=50, rtol=rtol)
+    assert ito == 4 and itg == ito
+    assert np.max(np.abs(xg - xo)) <= 1e-9 * np.abs(xo).max()
+    assert abs(rsg - rso) <= 1e-6 * rso
+
+
 # energy moments at sample counts that give every range layout of the
 # two-pass reduction: 50 bins of 120 (one range per bin), 32 bins of 2048
 # (two ranges per bin), 2 bins of 1031 (a bin with no divisor in 64..1024:
