@@ -235,6 +235,25 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
 
 
+def sampler_only(spc):
+    """Sampling epochs alone on the default workload's chains and model with
+    spc kept samples per chain (for counter runs under rocprofv3 --pmc: the
+    difference of two spc values isolates the kept-sample local energies)."""
+    import rbmgpu
+
+    m = rbmgpu.Model(spec(WORKLOAD["chains"], spc))
+    g = m.gpu(precision=WORKLOAD["precision"])
+    g.set_params(m.initial_weights(421390484))
+    ms = []
+    for e in range(3):
+        t0 = time.perf_counter()
+        g.sample_epoch(epoch=e)
+        g.synchronize()
+        ms.append(1e3 * (time.perf_counter() - t0))
+    print(json.dumps({"sampler_only": {"samples_per_chain": spc, "chains": WORKLOAD["chains"],
+                                       "epoch_ms": ms}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,9 +263,14 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the plain vs learned proposal comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--sampler-only", type=int, default=0, metavar="SPC",
+                    help="only run three sampling epochs with SPC kept samples per chain")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.sampler_only:
+        sampler_only(a.sampler_only)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box here has one GPU, and the "
