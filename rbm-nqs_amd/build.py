@@ -26,7 +26,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare"]
 
-GPU_SOURCES = ["kern_f32.hip", "kern_f64.hip", "kern_struct.hip", "kern_learn.hip", "rbmgpu_host.hip"]
+GPU_SOURCES = ["kern_f32.hip", "kern_f64.hip", "kern_struct.hip", "kern_learn.hip", "kern_eloc.hip",
+               "rbmgpu_host.hip"]
 HOST_SOURCES = ["lattice.cpp", "model.cpp", "ini.cpp", "setup.cpp", "capi.cpp"]
 DRIVER_SOURCES = ["main.cpp", "adapters.cpp"]
 

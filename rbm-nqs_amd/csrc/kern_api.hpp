@@ -58,6 +58,7 @@ struct SampArgs {
     uint64_t thr;  // bond flip if philox word < thr (thr = bond_flips * 2^32)
     uint32_t epoch, k0, k1;
     int pf_refresh;  // fresh Pfaffian inverse every pf_refresh accepted moves (0: never)
+    int obs_later;   // 1: kept samples store state and tanh only, k_eloc adds the observables
     const int32_t* pb;
     uint64_t* states;
     double* obs;
@@ -66,6 +67,23 @@ struct SampArgs {
     uint32_t* acc;
     uint32_t* prop;
 };
+
+// local observables of stored samples (k_eloc): FP32 basic RBM, terms of
+// at most 4 sites, N x H tables (cosh 2W, sinh 2W at [f * H + j])
+constexpr int kElocThreads = 256;
+struct ElocArgs {
+    int64_t n;  // samples
+    int N, H, nw;
+    Terms T;
+    const cplx<float>* C;
+    const cplx<float>* S;
+    const cplx<float>* rb;
+    const uint64_t* states;  // n x nw
+    const cplx<float>* tanh;  // n x H
+    double* obs;              // n x 2 n_groups
+};
+size_t eloc_lds_bytes(int N, int H);
+hipError_t launch_eloc(const ElocArgs& a, int blocks, hipStream_t st);
 
 template <typename R>
 struct EvalArgs {

@@ -722,7 +722,7 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
 #pragma unroll
             for (int u = 0; u < K; u++)
                 if (U.a[u] >= 0) a.Tout[smp * d.H + ln + u * L] = U.tn[u];
-            eval_obs<R, L, K, PF>(d, l, U, pv, st, ln, a.obs + smp * 2 * a.T.n_groups);
+            if (!a.obs_later) eval_obs<R, L, K, PF>(d, l, U, pv, st, ln, a.obs + smp * 2 * a.T.n_groups);
             if constexpr (PF) pf_derivative(l.pf, pv, st, ln, a.Opf + smp * 4 * d.pf_rows);
             k++;
         }

@@ -26,7 +26,7 @@ struct rbmgpu_ctx {
     int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0, pf_refresh = 0;
     int64_t P = 0, P_rbm = 0, ld = 0;  // ld: O row length (P rounded up to even)
     std::vector<int32_t> perm, basis, pb;
-    int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0;
+    int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0, tk_max = 0;
     // ---- sampler
     int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
     int64_t n_samples = 0;
@@ -297,11 +297,40 @@ SampArgs<R> samp_args(rbmgpu_ctx* c, uint32_t epoch, int block, int64_t n_sample
     return a;
 }
 
+// kept samples' local observables by k_eloc after the sampler (FP32 basic
+// RBM, terms of at most 4 sites, tables in LDS) instead of inside it
+bool eloc_after(const rbmgpu_ctx* c) {
+    return c->prec == RBMGPU_FP32 && c->pf_rows == 0 && c->nsymm == 1 && c->n_terms >= 1 &&
+           c->n_terms <= kElocThreads && c->tk_max <= 4 && c->nw <= 2 && eloc_lds_bytes(c->N, c->H) <= 65536;
+}
+
 template <typename R>
 int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
     rbmgpu_ctx* ctx = c;
     SampArgs<R> a = samp_args<R>(c, epoch, block, n_samples, warmup);
+    a.obs_later = eloc_after(c);
     HIPCK(launch_sample<R>(a, s, block, c->stream));
+    return 0;
+}
+
+int run_eloc(rbmgpu_ctx* c, int64_t n) {
+    rbmgpu_ctx* ctx = c;
+    ElocArgs a{};
+    a.n = n;
+    a.N = c->N;
+    a.H = c->H;
+    a.nw = c->nw;
+    a.T = terms(c);
+    Tabs<float> t = tabs<float>(c);
+    a.C = t.C;
+    a.S = t.S;
+    a.rb = t.rb;
+    a.states = c->d_states;
+    a.tanh = (const cplx<float>*)c->d_T;
+    a.obs = c->d_obs;
+    const int64_t per = kElocThreads / c->n_terms;
+    const int64_t blocks = std::min<int64_t>((n + per - 1) / per, 4 * (int64_t)c->n_cu);
+    HIPCK(launch_eloc(a, (int)blocks, c->stream));
     return 0;
 }
 
@@ -1242,6 +1271,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (upload(ctx, &c->d_pb, c->pb.data(), c->pb.size() * 4)) return bad(c->err);
     if (c->pf_rows && upload(ctx, &c->d_bs, bs.data(), bs.size() * 2)) return bad(c->err);
     if (upload(ctx, &c->d_tk, tk.data(), tk.size() * 4)) return bad(c->err);
+    c->tk_max = tk.empty() ? 0 : *std::max_element(tk.begin(), tk.end());
     if (upload(ctx, &c->d_tgroup, tgroup.data(), tgroup.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_rowbase, rowbase.data(), rowbase.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_rowptr, rowptr.data(), rowptr.size() * 4)) return bad(c->err);
@@ -1366,6 +1396,7 @@ int rbmgpu_sample_epoch(rbmgpu_ctx* c, uint32_t epoch) {
     } else {
         rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, epoch, c->shape, c->block, c->n_samples, c->warmup)
                                     : run_sample<double>(c, epoch, c->shape, c->block, c->n_samples, c->warmup);
+        if (!rc && eloc_after(c)) rc = run_eloc(c, c->n_samples);
     }
     if (rc) return rc;
     HIPCK(hipEventRecord(c->ev[1], c->stream));

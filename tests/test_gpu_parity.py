@@ -256,6 +256,20 @@ def test_sampler_trajectory_matches_oracle_fp64(name):
     assert np.allclose(acc, acc_o, rtol=0, atol=1e-15)
 
 
+# the kept samples' local observables of an FP32 sampling epoch against the
+# oracle's on the same states: basic RBMs with terms of at most 4 sites go
+# through k_eloc after the sampler, the others through the sampler's own
+# evaluation (local_op.cpp:31-65)
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(SPECS))
+def test_sampler_fp32_local_energies(name):
+    m, o, g, rng = _setup(SPECS[name], 32, std=0.1)
+    g.sample_epoch(epoch=0)
+    st, obs, _, _ = g.get_samples(states=True, obs=True, O=False)
+    e_ref = o.eloc(st)
+    assert np.max(np.abs(obs[:, 0] - e_ref)) <= 1e-4 * (np.abs(e_ref).max() + 1)
+
+
 def test_sampler_fp32_matches_exact_distribution():
     spec = ini("kitaev", 2, extra="""
 [sampler]
