@@ -81,6 +81,9 @@ struct ElocArgs {
     const uint64_t* states;  // n x nw
     const cplx<float>* tanh;  // n x H
     double* obs;              // n x 2 n_groups
+    int n_ratio;              // terms with an off-diagonal entry
+    const int32_t* ratio_term;  // [n_ratio] their indices, in order
+    const int32_t* term_slot;   // [n_terms] index into ratio_term, or -1 (diagonal only)
 };
 size_t eloc_lds_bytes(int N, int H);
 hipError_t launch_eloc(const ElocArgs& a, int blocks, hipStream_t st);

@@ -38,8 +38,9 @@ __global__ void __launch_bounds__(kElocThreads) k_eloc(ElocArgs a) {
         St[i] = a.S[f * H + j];
     }
     __syncthreads();
-    const int per = kElocThreads / nt;  // samples per pass of the block
-    const int q = threadIdx.x / nt, t = threadIdx.x - q * nt;
+    const int nr = a.n_ratio;             // terms with an off-diagonal entry: one thread each
+    const int per = kElocThreads / nr;  // samples per pass of the block
+    const int q = threadIdx.x / nr, t = a.ratio_term[threadIdx.x - q * nr];
     for (int64_t s0 = (int64_t)blockIdx.x * per; s0 < a.n; s0 += (int64_t)gridDim.x * per) {
         const int64_t smp = s0 + q;
         const bool live = q < per && smp < a.n;
@@ -78,6 +79,7 @@ __global__ void __launch_bounds__(kElocThreads) k_eloc(ElocArgs a) {
                 // sites not flipped enter as cosh = 1, sinh = 0 (exact)
                 cplx<float> p{1.f, 0.f};
                 int pe = 0;
+#pragma unroll 2
                 for (int j = 0; j < H; j++) {
                     cplx<float> ch{1.f, 0.f}, sh{0.f, 0.f};
 #pragma unroll
@@ -105,8 +107,10 @@ __global__ void __launch_bounds__(kElocThreads) k_eloc(ElocArgs a) {
         part[2 * threadIdx.x] = cr;
         part[2 * threadIdx.x + 1] = ci;
         __syncthreads();
-        if (live && t == 0) {
-            // the sample's group sums, terms in order
+        if (live && threadIdx.x - q * nr == 0) {
+            // the sample's group sums, terms in order: a ratio term's sum from
+            // its thread, a diagonal term's entries here
+            uint64_t st[2] = {a.states[smp * a.nw], a.nw > 1 ? a.states[smp * a.nw + 1] : 0};
             double* out = a.obs + smp * 2 * a.T.n_groups;
             int g = a.T.group[0];
             double gr = 0, gi = 0;
@@ -118,8 +122,20 @@ __global__ void __launch_bounds__(kElocThreads) k_eloc(ElocArgs a) {
                     gr = gi = 0;
                     g = gt;
                 }
-                gr += part[2 * (q * nt + ti)];
-                gi += part[2 * (q * nt + ti) + 1];
+                const int slot = a.term_slot[ti];
+                if (slot >= 0) {
+                    gr += part[2 * (q * nr + slot)];
+                    gi += part[2 * (q * nr + slot) + 1];
+                } else {
+                    const int k = a.T.k[ti];
+                    int loc = 0;
+                    for (int i = 0; i < k; i++) loc |= state_bit(st, a.T.sites[ti * 8 + i]) << i;
+                    const int base = a.T.rowbase[ti] + loc;
+                    for (int e = a.T.rowptr[base]; e < a.T.rowptr[base + 1]; e++) {
+                        gr += a.T.val[2 * e];
+                        gi += a.T.val[2 * e + 1];
+                    }
+                }
             }
             out[2 * g] = gr;
             out[2 * g + 1] = gi;
@@ -134,7 +150,7 @@ size_t eloc_lds_bytes(int N, int H) { return (size_t)N * H * 16 + (size_t)kElocT
 
 hipError_t launch_eloc(const ElocArgs& a, int blocks, hipStream_t st) {
     const size_t lds = eloc_lds_bytes(a.N, a.H);
-    if (a.T.n_terms < 1 || a.T.n_terms > kElocThreads || lds > 65536 || a.nw < 1 || a.nw > 2)
+    if (a.T.n_terms < 1 || a.n_ratio < 1 || a.n_ratio > kElocThreads || lds > 65536 || a.nw < 1 || a.nw > 2)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_eloc, dim3(blocks), dim3(kElocThreads), lds, st, a);
     return hipGetLastError();

@@ -26,7 +26,7 @@ struct rbmgpu_ctx {
     int N = 0, nvb = 0, na = 0, nsymm = 1, H = 0, nw = 1, pf_rows = 0, pf_refresh = 0;
     int64_t P = 0, P_rbm = 0, ld = 0;  // ld: O row length (P rounded up to even)
     std::vector<int32_t> perm, basis, pb;
-    int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0, tk_max = 0;
+    int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0, tk_max = 0, n_ratio = 0;
     // ---- sampler
     int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
     int64_t n_samples = 0;
@@ -44,6 +44,7 @@ struct rbmgpu_ctx {
     void* d_fs = nullptr;     // Pfaffian weights in the weight block
     int32_t *d_invperm = nullptr, *d_vbsite = nullptr, *d_pb = nullptr;
     int32_t *d_tk = nullptr, *d_tgroup = nullptr, *d_rowbase = nullptr, *d_rowptr = nullptr;
+    int32_t *d_ratio_term = nullptr, *d_term_slot = nullptr;
     uint8_t *d_tsites = nullptr, *d_mask = nullptr;
     double* d_val = nullptr;
     // samples
@@ -300,8 +301,8 @@ SampArgs<R> samp_args(rbmgpu_ctx* c, uint32_t epoch, int block, int64_t n_sample
 // kept samples' local observables by k_eloc after the sampler (FP32 basic
 // RBM, terms of at most 4 sites, tables in LDS) instead of inside it
 bool eloc_after(const rbmgpu_ctx* c) {
-    return c->prec == RBMGPU_FP32 && c->pf_rows == 0 && c->nsymm == 1 && c->n_terms >= 1 &&
-           c->n_terms <= kElocThreads && c->tk_max <= 4 && c->nw <= 2 && eloc_lds_bytes(c->N, c->H) <= 65536;
+    return c->prec == RBMGPU_FP32 && c->pf_rows == 0 && c->nsymm == 1 && c->n_ratio >= 1 &&
+           c->n_ratio <= kElocThreads && c->tk_max <= 4 && c->nw <= 2 && eloc_lds_bytes(c->N, c->H) <= 65536;
 }
 
 template <typename R>
@@ -328,7 +329,10 @@ int run_eloc(rbmgpu_ctx* c, int64_t n) {
     a.states = c->d_states;
     a.tanh = (const cplx<float>*)c->d_T;
     a.obs = c->d_obs;
-    const int64_t per = kElocThreads / c->n_terms;
+    a.n_ratio = c->n_ratio;
+    a.ratio_term = c->d_ratio_term;
+    a.term_slot = c->d_term_slot;
+    const int64_t per = kElocThreads / c->n_ratio;
     const int64_t blocks = std::min<int64_t>((n + per - 1) / per, 4 * (int64_t)c->n_cu);
     HIPCK(launch_eloc(a, (int)blocks, c->stream));
     return 0;
@@ -1217,8 +1221,10 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     std::vector<int32_t> tk, tgroup, rowbase, rowptr;
     std::vector<uint8_t> tsites, mask;
     std::vector<double> val;
+    std::vector<int32_t> ratio_term, term_slot;  // terms with an off-diagonal entry
     int64_t so = 0, mo = 0;
     for (int t = 0; t < c->n_terms; t++) {
+        const size_t m0 = mask.size();
         int k = cfg->term_k[t];
         if (k < 1 || k > 8) return bad("term with more than 8 sites");
         int g = cfg->term_group ? cfg->term_group[t] : 0;
@@ -1246,7 +1252,11 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
         }
         rowptr.push_back((int32_t)mask.size());
         mo += (int64_t)d * d;
+        const bool off = std::any_of(mask.begin() + m0, mask.end(), [](uint8_t x) { return x != 0; });
+        term_slot.push_back(off ? (int32_t)ratio_term.size() : -1);
+        if (off) ratio_term.push_back(t);
     }
+    c->n_ratio = (int)ratio_term.size();
     // ---- device
     int dev = 0;
     HIPCK(hipGetDevice(&dev));
@@ -1271,6 +1281,8 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (upload(ctx, &c->d_pb, c->pb.data(), c->pb.size() * 4)) return bad(c->err);
     if (c->pf_rows && upload(ctx, &c->d_bs, bs.data(), bs.size() * 2)) return bad(c->err);
     if (upload(ctx, &c->d_tk, tk.data(), tk.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_ratio_term, ratio_term.data(), ratio_term.size() * 4)) return bad(c->err);
+    if (upload(ctx, &c->d_term_slot, term_slot.data(), term_slot.size() * 4)) return bad(c->err);
     c->tk_max = tk.empty() ? 0 : *std::max_element(tk.begin(), tk.end());
     if (upload(ctx, &c->d_tgroup, tgroup.data(), tgroup.size() * 4)) return bad(c->err);
     if (upload(ctx, &c->d_rowbase, rowbase.data(), rowbase.size() * 4)) return bad(c->err);
@@ -1337,7 +1349,7 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,
                     c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,
                     c->d_rec_k,  c->d_rec_a,  c->d_lacc,   c->d_lprop, c->d_tr,     c->d_tst,    c->d_tseed,
-                    c->d_tw};
+                    c->d_tw,     c->d_ratio_term, c->d_term_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev)
