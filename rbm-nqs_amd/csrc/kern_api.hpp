@@ -70,7 +70,7 @@ struct SampArgs {
 
 // local observables of stored samples (k_eloc): FP32 basic RBM, terms of
 // at most 4 sites, N x H tables (cosh 2W, sinh 2W at [f * H + j])
-constexpr int kElocThreads = 256;
+constexpr int kElocThreads = 512;  // one block's LDS tables serve 8 wavefronts
 struct ElocArgs {
     int64_t n;  // samples
     int N, H, nw;

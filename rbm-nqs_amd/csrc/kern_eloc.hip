@@ -79,7 +79,6 @@ __global__ void __launch_bounds__(kElocThreads) k_eloc(ElocArgs a) {
                 // sites not flipped enter as cosh = 1, sinh = 0 (exact)
                 cplx<float> p{1.f, 0.f};
                 int pe = 0;
-#pragma unroll 2
                 for (int j = 0; j < H; j++) {
                     cplx<float> ch{1.f, 0.f}, sh{0.f, 0.f};
 #pragma unroll
