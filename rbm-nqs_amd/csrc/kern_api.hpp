@@ -86,6 +86,7 @@ struct ElocArgs {
     const int32_t* term_slot;   // [n_terms] index into ratio_term, or -1 (diagonal only)
 };
 size_t eloc_lds_bytes(int N, int H);
+int eloc_blocks_per_cu(int N, int H);
 hipError_t launch_eloc(const ElocArgs& a, int blocks, hipStream_t st);
 
 template <typename R>

@@ -147,6 +147,15 @@ __global__ void __launch_bounds__(kElocThreads) k_eloc(ElocArgs a) {
 
 size_t eloc_lds_bytes(int N, int H) { return (size_t)N * H * 16 + (size_t)kElocThreads * 16; }
 
+// resident k_eloc blocks per CU (the grid is sized to one full wave of
+// them: its grid-stride loop then gives every block the same share)
+int eloc_blocks_per_cu(int N, int H) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_eloc, kElocThreads, eloc_lds_bytes(N, H)) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
 hipError_t launch_eloc(const ElocArgs& a, int blocks, hipStream_t st) {
     const size_t lds = eloc_lds_bytes(a.N, a.H);
     if (a.T.n_terms < 1 || a.n_ratio < 1 || a.n_ratio > kElocThreads || lds > 65536 || a.nw < 1 || a.nw > 2)

@@ -27,6 +27,7 @@ struct rbmgpu_ctx {
     int64_t P = 0, P_rbm = 0, ld = 0;  // ld: O row length (P rounded up to even)
     std::vector<int32_t> perm, basis, pb;
     int n_terms = 0, n_groups = 1, n_rowptr = 0, n_ent = 0, tk_max = 0, n_ratio = 0;
+    int eloc_bpc = 0;  // resident k_eloc blocks per CU (0: not yet queried)
     // ---- sampler
     int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
     int64_t n_samples = 0;
@@ -333,7 +334,8 @@ int run_eloc(rbmgpu_ctx* c, int64_t n) {
     a.ratio_term = c->d_ratio_term;
     a.term_slot = c->d_term_slot;
     const int64_t per = kElocThreads / c->n_ratio;
-    const int64_t blocks = std::min<int64_t>((n + per - 1) / per, 4 * (int64_t)c->n_cu);
+    if (!c->eloc_bpc) c->eloc_bpc = eloc_blocks_per_cu(c->N, c->H);
+    const int64_t blocks = std::min<int64_t>((n + per - 1) / per, (int64_t)c->eloc_bpc * c->n_cu);
     HIPCK(launch_eloc(a, (int)blocks, c->stream));
     return 0;
 }
