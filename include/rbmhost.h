@@ -35,6 +35,10 @@ int rbmhost_clusters(const rbmhost_model* m, int64_t* out /* N x 8 */, int64_t* 
 /* sites (k each, concatenated) and 2^k x 2^k matrices of the terms */
 int rbmhost_terms(const rbmhost_model* m, int64_t* k, int64_t* sites, double* mats, int64_t* groups);
 double rbmhost_exact_energy(const rbmhost_model* m);
+/* the Pfaffian index map bs_ (pfaffian.cpp:31-85): N x N column-major,
+ * A_ij = sgn(bs_ij) fs_(|bs_ij| - 1, 2 s_i + s_j); rows = fs_ rows.
+ * Returns -1 if the machine has no Pfaffian factor. */
+int rbmhost_pf_map(const rbmhost_model* m, int64_t* out, int64_t* rows);
 
 /* switch observables to the --evaluate set (main.cpp:640-680): helper
  * terms removed, plus one group per hexagon plaquette (Kitaev) */
@@ -45,6 +49,11 @@ int rbmhost_set_n_samples(rbmhost_model* m, int64_t n);
 /* weights drawn like rbm_base::initialize_weights (std::mt19937(seed),
  * normal_distribution) followed by the Pfaffian weights */
 int rbmhost_initial_weights(const rbmhost_model* m, uint64_t seed, double* params);
+
+/* the massive-step guard of abstract_optimizer::optimize
+ * (abstract_optimizer.cpp:157): 1 if max |dw_i| <= 30 and every entry is
+ * finite, else 0 (the driver then exits with rc 55).  dw: n complex. */
+int rbmhost_step_ok(const double* dw, int64_t n, double* max_abs);
 
 /* GPU context for this model; precision / lanes / chains / samples of 0
  * keep the .ini values */

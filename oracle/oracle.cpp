@@ -1222,6 +1222,13 @@ int orc_plaqs(const orc_model* m, int64_t* out, int64_t* n) {
     *n = pl.size();
     return 0;
 }
+int orc_pf_map(const orc_model* m, int64_t* out, int64_t* rows) {
+    const Pfaffian* pf = m->s.mach.pf.get();
+    if (!pf) return -1;
+    for (size_t i = 0; i < pf->bs.size(); i++) out[i] = pf->bs[i];
+    *rows = (int64_t)pf->rows;
+    return 0;
+}
 int orc_symmetry(const orc_model* m, int64_t* perms) {
     const Machine& mc = m->s.mach;
     for (size_t s = 0; s < mc.n_symm; s++)

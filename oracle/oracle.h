@@ -43,6 +43,8 @@ int orc_nns(const orc_model* m, int64_t site, int64_t* out /* n_coord */);
 int orc_hexagons(const orc_model* m, int64_t* out /* 6*n_uc_total */, int64_t* n);
 int orc_plaqs(const orc_model* m, int64_t* out /* 5*n: 4 idx + type */, int64_t* n);
 int orc_symmetry(const orc_model* m, int64_t* perms /* n_symm*N */);
+/* Pfaffian index map bs_ (N x N column-major) and fs_ rows; -1 without a Pfaffian */
+int orc_pf_map(const orc_model* m, int64_t* out, int64_t* rows);
 int orc_up_down(const orc_model* m, int64_t uc, int64_t dir, int64_t step,
                 int64_t* up, int64_t* down);
 int orc_bravais_updown(int64_t n_uc, int64_t n_uc_b, int64_t h_shift, int64_t uc,

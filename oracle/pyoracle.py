@@ -114,6 +114,13 @@ class Oracle:
         _chk(lib().orc_symmetry(self.h, _p(out, I64P)))
         return out.reshape(self.n_symm, self.N)
 
+    def pf_map(self):
+        """Pfaffian index map bs_ (N x N, bs[i, j]) and the number of fs_ rows"""
+        out = np.zeros(self.N * self.N, dtype=np.int64)
+        rows = C.c_int64()
+        _chk(lib().orc_pf_map(self.h, _p(out, I64P), C.byref(rows)))
+        return out.reshape(self.N, self.N).T.copy(), rows.value
+
     def exact_energy(self):
         return lib().orc_exact_energy(self.h)
 

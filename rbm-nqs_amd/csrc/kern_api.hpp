@@ -125,8 +125,9 @@ struct MomArgs {
     int P, n_groups, ld;
     const cplx<R>* O;
     const double* obs;
-    double* part;
+    double* part;  // blocks x P x kOPartRow (k_o_partial)
 };
+constexpr int kOPartRow = 7;
 
 struct CgStatus {
     double rsold, rs;

@@ -151,8 +151,10 @@ size_t eloc_lds_bytes(int N, int H) { return (size_t)N * H * 16 + (size_t)kElocT
 // them: its grid-stride loop then gives every block the same share)
 int eloc_blocks_per_cu(int N, int H) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_eloc, kElocThreads, eloc_lds_bytes(N, H)) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_eloc, kElocThreads, eloc_lds_bytes(N, H)) != hipSuccess) {
+        (void)hipGetLastError();  // clear it: the launch that follows reports its own status
         return 1;
+    }
     return nb > 0 ? nb : 1;
 }
 

@@ -843,26 +843,36 @@ __global__ void __launch_bounds__(256) k_build_O(OArgs<R> a) {
 // sum E conj(O) (accumulated in double)
 template <typename R>
 __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
+    // per block of samples: sums of O - K, |O - K|^2 with the shift K = the
+    // block's first O (shifted-data variance: no cancellation when O is
+    // nearly constant, e.g. saturated hidden units) and of E conj(O).
+    // part row per (block, p): [K re, K im, sum d re, sum d im, sum |d|^2,
+    // sum E conj(O) re, im]; k_o_final merges the blocks pairwise.
     int64_t k0 = (int64_t)blockIdx.x * a.chunk, k1 = k0 + a.chunk;
     if (k1 > a.S) k1 = a.S;
     for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
+        const cplx<R> o0 = a.O[k0 * a.ld + p];
+        const double kr = o0.re, ki = o0.im;
         double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0;
         for (int64_t k = k0; k < k1; k++) {
             cplx<R> o = a.O[k * a.ld + p];
             double ore = o.re, oim = o.im;
             double ere = a.obs[k * 2 * a.n_groups], eim = a.obs[k * 2 * a.n_groups + 1];
-            s0 += ore;
-            s1 += oim;
-            s2 += ore * ore + oim * oim;
+            const double dr = ore - kr, di = oim - ki;
+            s0 += dr;
+            s1 += di;
+            s2 += dr * dr + di * di;
             e0 += ere * ore + eim * oim;  // E * conj(O)
             e1 += eim * ore - ere * oim;
         }
-        double* q = a.part + ((int64_t)blockIdx.x * a.P + p) * 5;
-        q[0] = s0;
-        q[1] = s1;
-        q[2] = s2;
-        q[3] = e0;
-        q[4] = e1;
+        double* q = a.part + ((int64_t)blockIdx.x * a.P + p) * kOPartRow;
+        q[0] = kr;
+        q[1] = ki;
+        q[2] = s0;
+        q[3] = s1;
+        q[4] = s2;
+        q[5] = e0;
+        q[6] = e1;
     }
 }
 

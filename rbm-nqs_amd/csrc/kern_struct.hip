@@ -45,8 +45,10 @@ template <int CT, int RT>
 static int occupancy() {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_struct<CT, RT>, StructCfg<CT, RT>::THREADS,
-                                                     lds_for(CT, RT)) != hipSuccess)
+                                                     lds_for(CT, RT)) != hipSuccess) {
+        (void)hipGetLastError();  // clear it: the launch that follows reports its own status
         return 1;
+    }
     return nb > 0 ? nb : 1;
 }
 

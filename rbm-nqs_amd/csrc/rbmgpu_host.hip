@@ -64,6 +64,8 @@ struct rbmgpu_ctx {
     double *d_part = nullptr, *d_omean = nullptr, *d_ovar = nullptr, *d_eo = nullptr,
            *d_maxvar = nullptr, *d_emom = nullptr, *d_epart = nullptr;
     int64_t epart_cap = 0;  // bytes of d_epart
+    double* d_varpart = nullptr;  // chunk partials of the structured Var(O)
+    int64_t varpart_cap = 0;
     int64_t part_cap = 0;
     bool moments_valid = false;
     // CG / S.x
@@ -110,6 +112,7 @@ struct rbmgpu_ctx {
     int64_t mlp_n() const { return (int64_t)Hn * N + Hn + (int64_t)N * Hn + N; }
     std::string err;
 };
+
 
 namespace {
 
@@ -568,23 +571,40 @@ float elapsed(rbmgpu_ctx* c, int a, int b) {
 }
 
 // ---- reductions in FP64 (aggregator.cpp:84-239, one rank, unit weights)
-__global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb, int64_t S, int P,
+__global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb, int64_t chunk, int64_t S, int P,
                                                  double* mean, double* var, double* eo,
                                                  unsigned long long* maxvar) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
-    double q[5] = {0, 0, 0, 0, 0};
-    for (int64_t b = 0; b < nb; b++)
-        for (int c = 0; c < 5; c++) q[c] += part[(b * P + p) * 5 + c];
-    double inv = 1.0 / (double)S;
-    double mr = q[0] * inv, mi = q[1] * inv;
+    // block b (n_b samples): mean_b = K_b + sum d / n_b, M2_b = sum |d|^2 -
+    // |sum d|^2 / n_b; merged pairwise, M2 = sum_b M2_b + n_b |mean_b -
+    // mean|^2 (Chan, Golub and LeVeque; the reference accumulates Welford
+    // updates, aggregator.cpp:190-231)
+    double sr = 0, si = 0, er = 0, ei = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        const double* q = part + (b * P + p) * kOPartRow;
+        const double n = (double)(b + 1 < nb ? chunk : S - b * chunk);
+        sr += n * q[0] + q[2];
+        si += n * q[1] + q[3];
+        er += q[5];
+        ei += q[6];
+    }
+    const double inv = 1.0 / (double)S;
+    const double mr = sr * inv, mi = si * inv;
+    double m2 = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        const double* q = part + (b * P + p) * kOPartRow;
+        const double n = (double)(b + 1 < nb ? chunk : S - b * chunk);
+        const double dr = q[2] / n, di = q[3] / n;                 // mean_b - K_b
+        const double ur = q[0] + dr - mr, ui = q[1] + di - mi;     // mean_b - mean
+        m2 += (q[4] - n * (dr * dr + di * di)) + n * (ur * ur + ui * ui);
+    }
     mean[2 * p] = mr;
     mean[2 * p + 1] = mi;
-    double v = (q[2] - (double)S * (mr * mr + mi * mi)) / ((double)S - 1.0);
-    v = fmax(v, 0.0);
+    const double v = S > 1 ? fmax(m2 / ((double)S - 1.0), 0.0) : 0.0;
     var[p] = v;
-    eo[2 * p] = q[3] * inv;
-    eo[2 * p + 1] = q[4] * inv;
+    eo[2 * p] = er * inv;
+    eo[2 * p + 1] = ei * inv;
     atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
 }
 
@@ -842,7 +862,7 @@ __global__ void __launch_bounds__(256) k_struct_moments(const double* A, const d
             im += A[2 * ((int64_t)c * PA + q) + 1];
         }
     };
-    double ar, ai, br = 0, bi = 0, t2, dummy;
+    double ar, ai, br = 0, bi = 0;
     sumA(p, ar, ai);
     for (int c = 0; c < nch; c++) {
         br += B[2 * ((int64_t)c * P + p)];
@@ -850,26 +870,130 @@ __global__ void __launch_bounds__(256) k_struct_moments(const double* A, const d
     }
     double inv = 1.0 / (double)S;
     double mr = ar * inv, mi = -ai * inv;  // mean = conj(sum conj(O)) / S
-    double so2;
-    if (p < N) {
-        so2 = (double)S;  // sigma^2 = 1
-    } else {
-        int i = p < N + M ? p - N : (p - N - M) / N;
-        sumA(P + i, t2, dummy);
-        so2 = t2;
-    }
-    double v = fmax((so2 - (double)S * (mr * mr + mi * mi)) / ((double)S - 1.0), 0.0);
     mean[2 * p] = mr;
     mean[2 * p + 1] = mi;
-    var[p] = v;
     eo[2 * p] = br * inv;
     eo[2 * p + 1] = bi * inv;
+    if (p < N) {
+        // sigma = +-1: sum_k |sigma_k - mean|^2 = S (1 - mean^2), where the
+        // sum of sigma is an exact integer
+        double v = S > 1 ? fmax((double)S * (1.0 - mr * mr) / ((double)S - 1.0), 0.0) : 0.0;
+        var[p] = v;
+        atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// Var(O) of the hidden and weight blocks of the structured basic RBM by a
+// second pass over the tanh rows, once the means are known: sum_k |O_kp -
+// mean_p|^2 with the deviation formed per sample (O = t_j for the hidden
+// block, sigma_i t_j for the weights), so saturated units with |<O>|^2 >>
+// Var(O) keep their digits.  Lane l = hidden unit 64 blockIdx.z + l;
+// blockIdx.y = spins 16y .. 16y+15; the kVarWaves waves of a block walk chunks of kVarChunk
+// samples in a grid-stride loop, the next chunk's loads issued before the
+// current chunk is used.  FP32 deviations and chunk sums, FP64 across
+// chunks; one partial row per block: [hidden j | W (i + j N)] (M + N M).
+constexpr int kVarSpins = 16, kVarChunk = 8, kVarWaves = 4;
+__global__ void __launch_bounds__(64 * kVarWaves) k_struct_var_part(const uint64_t* states, const float2* T,
+                                                                    int64_t S, int N, int M, int nw,
+                                                                    const double* mean, double* part) {
+    __shared__ double red[kVarWaves - 1][kVarSpins + 1][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j0 = lane + 64 * (int)blockIdx.z;  // hidden units 64 z .. 64 z + 63
+    const int j = j0 < M ? j0 : M - 1;           // lanes past M repeat unit M-1 and write nothing
+    const int i0 = blockIdx.y * kVarSpins;
+    const int ni = N - i0 < kVarSpins ? N - i0 : kVarSpins;
+    float mr[kVarSpins], mi[kVarSpins];
+    double a64[kVarSpins], h64 = 0;
+#pragma unroll
+    for (int q = 0; q < kVarSpins; q++) {
+        const int p = N + M + (q < ni ? i0 + q : i0) + j * N;
+        mr[q] = (float)mean[2 * p];
+        mi[q] = (float)mean[2 * p + 1];
+        a64[q] = 0;
+    }
+    const float hr = (float)mean[2 * (N + j)], hi = (float)mean[2 * (N + j) + 1];
+    const int word = i0 >> 6, shift = i0 & 63;  // 16 spins never straddle a word
+    const int64_t nch = (S + kVarChunk - 1) / kVarChunk, stride = (int64_t)gridDim.x * kVarWaves;
+    float2 tc[kVarChunk];
+    uint32_t bc[kVarChunk];
+    auto fetch = [&](int64_t c, float2 (&t)[kVarChunk], uint32_t (&b)[kVarChunk]) {
+#pragma unroll
+        for (int r = 0; r < kVarChunk; r++) {
+            const int64_t k0 = c * kVarChunk + r, k = k0 < S ? k0 : S - 1;  // clamped, masked below
+            t[r] = T[k * M + j];
+            b[r] = (uint32_t)(states[k * nw + word] >> shift);
+        }
+    };
+    int64_t c = (int64_t)blockIdx.x * kVarWaves + w;
+    if (c < nch) fetch(c, tc, bc);
+    while (c < nch) {
+        const int64_t cn = c + stride;
+        float2 tn[kVarChunk];
+        uint32_t bn[kVarChunk];
+        if (cn < nch) fetch(cn, tn, bn);
+        const int64_t nr = S - c * kVarChunk;
+        float a32[kVarSpins], h32 = 0.f;
+#pragma unroll
+        for (int q = 0; q < kVarSpins; q++) a32[q] = 0.f;
+#pragma unroll
+        for (int r = 0; r < kVarChunk; r++) {
+            if (r < nr) {
+                const float2 t = tc[r];
+#pragma unroll
+                for (int q = 0; q < kVarSpins; q++) {
+                    const float sg = (bc[r] >> q) & 1u ? 1.f : -1.f;  // -sigma
+                    const float dr = fmaf(sg, mr[q], t.x), di = fmaf(sg, mi[q], t.y);
+                    a32[q] = fmaf(dr, dr, fmaf(di, di, a32[q]));
+                }
+                const float dr = t.x - hr, di = t.y - hi;
+                h32 = fmaf(dr, dr, fmaf(di, di, h32));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kVarSpins; q++) a64[q] += a32[q];
+        h64 += h32;
+#pragma unroll
+        for (int r = 0; r < kVarChunk; r++) {
+            tc[r] = tn[r];
+            bc[r] = bn[r];
+        }
+        c = cn;
+    }
+    // the block's waves add their sums in LDS
+    if (w > 0) {
+#pragma unroll
+        for (int q = 0; q < kVarSpins; q++) red[w - 1][q][lane] = a64[q];
+        red[w - 1][kVarSpins][lane] = h64;
+    }
+    __syncthreads();
+    if (w == 0 && j0 < M) {
+        for (int v = 0; v < kVarWaves - 1; v++) {
+#pragma unroll
+            for (int q = 0; q < kVarSpins; q++) a64[q] += red[v][q][lane];
+            h64 += red[v][kVarSpins][lane];
+        }
+        double* row = part + (int64_t)blockIdx.x * (M + (int64_t)N * M);
+        if (blockIdx.y == 0) row[j] = h64;
+#pragma unroll
+        for (int q = 0; q < kVarSpins; q++)
+            if (q < ni) row[M + i0 + q + (int64_t)j * N] = a64[q];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_struct_var_final(const double* part, int64_t nchunk, int64_t S, int N,
+                                                          int M, double* var, unsigned long long* maxvar) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, L = M + (int64_t)N * M;
+    if (q >= L) return;
+    double s = 0;
+    for (int64_t c = 0; c < nchunk; c++) s += part[c * L + q];
+    const double v = S > 1 ? s / ((double)S - 1.0) : 0.0;
+    var[N + q] = v;
     atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
 }
 
 int ensure_sx(rbmgpu_ctx* c);
 int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hipEvent_t e0 = nullptr,
-               hipEvent_t e1 = nullptr);
+               hipEvent_t e1 = nullptr, void* part = nullptr);
 
 int struct_moments(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
@@ -897,6 +1021,27 @@ int struct_moments(rbmgpu_ctx* c) {
                        c->S, c->N, c->na, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
                        (unsigned long long*)c->d_maxvar);
     HIPCK(hipGetLastError());
+    // second pass: Var(O) of the hidden and weight blocks
+    const int64_t L = c->na + (int64_t)c->N * c->na;
+    const int64_t nz = (c->na + 63) / 64;
+    // about 3 waves per SIMD over the spin groups, at most one chunk per wave
+    const int64_t ny = (c->N + kVarSpins - 1) / kVarSpins;
+    const int64_t vch = (c->S + kVarChunk - 1) / kVarChunk;
+    const int64_t nvb = std::max<int64_t>(
+        1, std::min<int64_t>((12 * (int64_t)c->n_cu + ny * nz * kVarWaves - 1) / (ny * nz * kVarWaves),
+                             (vch + kVarWaves - 1) / kVarWaves));
+    const int64_t vneed = nvb * L * (int64_t)sizeof(double);
+    if (vneed > c->varpart_cap) {
+        if (int rc = dalloc(ctx, &c->d_varpart, vneed)) return rc;
+        c->varpart_cap = vneed;
+    }
+    hipLaunchKernelGGL(k_struct_var_part, dim3((unsigned)nvb, (unsigned)ny, (unsigned)nz), dim3(64 * kVarWaves), 0,
+                       c->stream,
+                       c->d_states, (const float2*)c->d_T, c->S, c->N, c->na, c->nw, c->d_omean, c->d_varpart);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_struct_var_final, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c->stream, c->d_varpart,
+                       nvb, c->S, c->N, c->na, c->d_ovar, (unsigned long long*)c->d_maxvar);
+    HIPCK(hipGetLastError());
     return 0;
 }
 
@@ -914,8 +1059,9 @@ int ensure_cg(rbmgpu_ctx* c) {
     return 0;
 }
 
-int ensure_sx(rbmgpu_ctx* c) {
-    rbmgpu_ctx* ctx = c;
+// S.x launch layout for the current sample set: block count and the byte
+// size of the block partials (blocks x row complex)
+int sx_layout(rbmgpu_ctx* c, int64_t* blocks_out, int64_t* bytes_out) {
     int64_t blocks, row;
     if (c->structured) {
         // chunks of 16 samples, one per wave step; the resident waves walk them
@@ -930,8 +1076,16 @@ int ensure_sx(rbmgpu_ctx* c) {
         blocks = std::min<int64_t>(std::max<int64_t>(c->S / 32, 1), 4 * (int64_t)c->n_cu);
         row = c->ld;
     }
+    *blocks_out = blocks;
+    *bytes_out = blocks * row * 2 * (int64_t)rsize(c);
+    return 0;
+}
+
+int ensure_sx(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    int64_t blocks, need;
+    if (int rc = sx_layout(c, &blocks, &need)) return rc;
     c->sx_blocks = blocks;
-    int64_t need = blocks * row * 2 * rsize(c);
     if (need > c->sxpart_cap) {
         if (int rc = dalloc(ctx, &c->d_sxpart, need)) return rc;
         c->sxpart_cap = need;
@@ -973,11 +1127,12 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
 // block partials (the dominant kernel), reduction, finish, optional CG step
 template <typename R>
 int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-           double* pdot, bool cg) {
+           double* pdot, bool cg, void* part) {
     rbmgpu_ctx* ctx = c;
     SxArgs<R> a = sx_args<R>(c, x, y, r1, r2, rd, st);
+    if (part) a.part = (cplx<R>*)part;
     if (c->structured) {
-        if (int rc = run_struct(c, 0, x, st)) return rc;
+        if (int rc = run_struct(c, 0, x, st, nullptr, nullptr, part)) return rc;
     } else {
         HIPCK(launch_sx<R>(a, (int)c->sx_blocks, c->stream));
     }
@@ -992,7 +1147,8 @@ int run_sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, doub
     return 0;
 }
 
-int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hipEvent_t e0, hipEvent_t e1) {
+int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hipEvent_t e0, hipEvent_t e1,
+               void* part) {
     rbmgpu_ctx* ctx = c;
     StructArgs a{};
     a.S = c->S;
@@ -1007,7 +1163,7 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hip
     a.T = (const cplx<float>*)c->d_T;
     a.obs = c->d_obs;
     a.x = x;
-    a.part = (cplx<float>*)c->d_sxpart;
+    a.part = (cplx<float>*)(part ? part : c->d_sxpart);
     a.status = st;
     HIPCK(launch_struct(a, (int)c->sx_blocks, c->stream, e0, e1));
     return 0;
@@ -1015,10 +1171,11 @@ int run_struct(rbmgpu_ctx* c, int mode, const double* x, const CgStatus* st, hip
 
 // pdot: optional per-column-block partials of x^H y for the CG update
 // cg: the CG step on the product follows (x = d_p, y = d_Ap, pdot given)
+// part: block-partial buffer (default: the context's own)
 int sx(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd, const CgStatus* st,
-       double* pdot = nullptr, bool cg = false) {
-    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, pdot, cg)
-                                  : run_sx<double>(c, x, y, r1, r2, rd, st, pdot, cg);
+       double* pdot = nullptr, bool cg = false, void* part = nullptr) {
+    return c->prec == RBMGPU_FP32 ? run_sx<float>(c, x, y, r1, r2, rd, st, pdot, cg, part)
+                                  : run_sx<double>(c, x, y, r1, r2, rd, st, pdot, cg, part);
 }
 
 int compute_moments(rbmgpu_ctx* c) {
@@ -1069,7 +1226,7 @@ int compute_moments(rbmgpu_ctx* c) {
             chunk = (c->S + blocks - 1) / blocks;
             blocks = (c->S + chunk - 1) / chunk;
         }
-        int64_t need = blocks * c->P * 5;
+        int64_t need = blocks * c->P * kOPartRow;
         if (need > c->part_cap) {
             if (int rc = dalloc(ctx, &c->d_part, need * sizeof(double))) return rc;
             c->part_cap = need;
@@ -1085,7 +1242,7 @@ int compute_moments(rbmgpu_ctx* c) {
         }
         HIPCK(hipMemsetAsync(c->d_maxvar, 0, sizeof(double), c->stream));
         hipLaunchKernelGGL(k_o_final, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->d_part,
-                           blocks, c->S, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
+                           blocks, chunk, c->S, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
                            (unsigned long long*)c->d_maxvar);
         HIPCK(hipGetLastError());
     }
@@ -1345,7 +1502,7 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
     void* ptrs[] = {c->d_W,      c->d_perm,
                     c->d_invperm, c->d_vbsite, c->d_pb,     c->d_tk,    c->d_tgroup, c->d_rowbase, c->d_rowptr,
                     c->d_tsites, c->d_mask,   c->d_val,    c->d_states, c->d_obs,   c->d_T,      c->d_O,
-                    c->d_acc,    c->d_prop,   c->d_part,   c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
+                    c->d_acc,    c->d_prop,   c->d_part,   c->d_varpart, c->d_omean, c->d_ovar,   c->d_eo,     c->d_maxvar,
                     c->d_emom,   c->d_epart,  c->d_x,      c->d_b,      c->d_r,     c->d_p,      c->d_Ap,     c->d_st,
                     c->d_sxpart, c->d_sxpart2, c->d_evst,   c->d_evout,  c->d_evobs, c->d_bs,
                     c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,

@@ -1,6 +1,8 @@
 // adapters.cpp -- see adapters.hpp.
 #include "adapters.hpp"
 
+#include "optim.hpp"
+
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -224,9 +226,7 @@ std::vector<std::complex<double>>& stochastic_reconfiguration::gradient(bool log
 bool stochastic_reconfiguration::optimize() {
     double lr = lr_.get();
     gradient(true);
-    double mx = 0;
-    for (const auto& v : dw_) mx = std::max(mx, std::abs(v));
-    if (mx > 30) return false;  // massive step guard (abstract_optimizer.cpp:157)
+    if (!optim::step_ok(dw_.data(), dw_.size())) return false;  // abstract_optimizer.cpp:157
     for (auto& v : dw_) v *= lr;
     rbm_.update_weights(dw_);
     return true;

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/rbmhost.h"
+#include "optim.hpp"
 #include "setup.hpp"
 
 struct rbmhost_model {
@@ -127,6 +128,20 @@ int rbmhost_gpu_create(const rbmhost_model* m, int32_t precision, int32_t lanes,
     if (precision) c.precision = precision;
     if (lanes) c.lanes_per_chain = lanes;
     return rbmgpu_create(&c, out);
+}
+
+int rbmhost_pf_map(const rbmhost_model* m, int64_t* out, int64_t* rows) {
+    if (!m || !out || !rows) return -1;
+    const auto& L = m->s->layout();
+    if (!L.pf_rows) return -1;
+    for (size_t i = 0; i < L.pf_bs.size(); i++) out[i] = L.pf_bs[i];
+    *rows = L.pf_rows;
+    return 0;
+}
+
+int rbmhost_step_ok(const double* dw, int64_t n, double* max_abs) {
+    if (!dw || n < 0) return -1;
+    return optim::step_ok((const std::complex<double>*)dw, (size_t)n, max_abs) ? 1 : 0;
 }
 
 }  // extern "C"

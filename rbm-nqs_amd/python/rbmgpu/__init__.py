@@ -65,8 +65,18 @@ def exported_symbols():
         "host": ["rbmhost_create", "rbmhost_destroy", "rbmhost_info", "rbmhost_bonds",
                  "rbmhost_perms", "rbmhost_terms", "rbmhost_exact_energy", "rbmhost_use_evaluation",
                  "rbmhost_set_n_samples", "rbmhost_initial_weights", "rbmhost_gpu_create",
-                 "rbmhost_clusters"],
+                 "rbmhost_clusters", "rbmhost_step_ok", "rbmhost_pf_map"],
     }
+
+
+def step_ok(dw):
+    """massive-step guard of the driver (abstract_optimizer.cpp:157): (ok, max |dw|)"""
+    dw = np.ascontiguousarray(dw, dtype=np.complex128)
+    mx = C.c_double()
+    rc = libs()[1].rbmhost_step_ok(dw.ctypes.data_as(DP), C.c_int64(dw.size), C.byref(mx))
+    if rc < 0:
+        raise RuntimeError("rbmhost_step_ok: bad arguments")
+    return rc == 1, mx.value
 
 
 def _p(a, t):
@@ -112,6 +122,14 @@ class Model:
         out = np.zeros(self.n_symm * self.N, dtype=np.int64)
         libs()[1].rbmhost_perms(self.h, _p(out, I64P))
         return out.reshape(self.n_symm, self.N)
+
+    def pf_map(self):
+        """Pfaffian index map bs_ (N x N, bs[i, j]) and the number of fs_ rows"""
+        out = np.zeros(self.N * self.N, dtype=np.int64)
+        rows = C.c_int64()
+        if libs()[1].rbmhost_pf_map(self.h, _p(out, I64P), C.byref(rows)) != 0:
+            raise RuntimeError("rbmhost_pf_map: no Pfaffian factor")
+        return out.reshape(self.N, self.N).T.copy(), rows.value
 
     def clusters(self):
         """cluster catalogue of the learned proposal (N x kmax)"""

@@ -95,3 +95,24 @@ def test_cluster_catalogue_matches_oracle(model, n):
     """learned-proposal clusters: the first lattice loop through each site"""
     spec = ini(model, n)
     assert np.array_equal(rbmgpu.Model(spec).clusters(), po.Oracle(spec).clusters())
+
+
+def test_massive_step_guard():
+    """abstract_optimizer.cpp:157: an update with max |dw| > 30 is refused
+    (the driver then returns rc 55); non-finite entries are refused too."""
+    dw = np.zeros(100, np.complex128)
+    dw[7] = 29.9 + 0.1j
+    assert rbmgpu.step_ok(dw)[0]
+    assert abs(rbmgpu.step_ok(dw)[1] - abs(dw[7])) < 1e-15
+    dw[7] = 30.0
+    assert rbmgpu.step_ok(dw)[0]
+    dw[7] = 30.0 + 1e-6
+    assert not rbmgpu.step_ok(dw)[0]
+    dw[7] = 0
+    assert rbmgpu.step_ok(dw) == (True, 0.0)
+    for bad in (np.nan, np.inf, complex(0, np.nan), complex(-np.inf, 0)):
+        dw[3] = bad
+        assert not rbmgpu.step_ok(dw)[0]
+    dw[3] = 0
+    dw[99] = np.nan  # the last entry counts as well
+    assert not rbmgpu.step_ok(dw)[0]
