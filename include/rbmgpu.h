@@ -134,6 +134,49 @@ int rbmgpu_reduce(rbmgpu_ctx* ctx, rbmgpu_moments* out, double* o_mean,
 int rbmgpu_sr_solve(rbmgpu_ctx* ctx, const double* b, double* x, double r1,
                     double r2, double rd, int64_t max_iter, double rtol,
                     int64_t* itn, double* rs);
+/* One whole SR epoch on the device (main.cpp:596-610: sampler->sample()
+ * then optimizer->optimize()): sampling, local energies, moments, F =
+ * <E O*> - <E><O>* (SR.cpp:102), the CG solve warm-started from the
+ * previous epoch's lr-scaled step (cg_solver.cpp:119-189, SR.cpp:57), the
+ * massive-step guard (abstract_optimizer.cpp:157; non-finite steps are
+ * refused too) and w -= lr x (rbm_base.cpp:117-143).  The decayed
+ * lr / reg values of this epoch (abstract_optimizer.cpp:39-48) are the
+ * caller's.  With graph = 1 the body is captured once in a HIP graph and
+ * replayed (no counterpart in the reference); the parameters then live on
+ * the device (rbmgpu_get_params reads them from there). */
+typedef struct rbmgpu_sr_params {
+    double lr, reg1, reg2, reg1_delta;
+    int64_t max_iter;  /* 0: 2P (cg_solver.cpp:48-56) */
+    double rtol;       /* 0: 1e-16 */
+    int32_t graph;     /* 1: replay a captured HIP graph, 0: launch eagerly */
+} rbmgpu_sr_params;
+typedef struct rbmgpu_epoch_stats {
+    double e_mean[2], e_var, e_var_binned, e_tau, e_std;
+    int64_t n_bins;
+    double acceptance;
+    int64_t cg_itn;
+    double cg_rs;
+    double max_dw;        /* max |x| of the solve (NaN if not finite) */
+    int32_t updated;      /* 0: the guard refused the step (driver rc 55) */
+    int32_t graph_replayed;
+    double ms;            /* host wall time of the call */
+} rbmgpu_epoch_stats;
+int rbmgpu_train_epoch(rbmgpu_ctx* ctx, uint32_t epoch, const rbmgpu_sr_params* p, rbmgpu_epoch_stats* out);
+
+/* the SR solver family (SR.cpp:51-72): RBMGPU_SOLVE_CG is the matrix-free
+ * CG of rbmgpu_sr_solve (the reference's "cg" and "minres",
+ * cg_solver.cpp:31-37); RBMGPU_SOLVE_CG_SINGLE builds S explicitly
+ * (cg_solver.cpp:274-299) and runs the same CG on it; RBMGPU_SOLVE_DIRECT
+ * builds S with the direct solver's multiplicative regularisation and
+ * solves it exactly (direct_solver.cpp:31-59; itn = 0, rs = |S x - b|^2).
+ * The explicit forms need the O matrix (not the structured S.x). */
+enum { RBMGPU_SOLVE_CG = 0, RBMGPU_SOLVE_CG_SINGLE = 1, RBMGPU_SOLVE_DIRECT = 2 };
+int rbmgpu_sr_solve_ex(rbmgpu_ctx* ctx, int32_t method, const double* b, double* x, double r1, double r2,
+                       double rd, int64_t max_iter, double rtol, int64_t* itn, double* rs);
+/* the explicit S of the current sample set with the CG regularisation of
+ * cg_solver.cpp:276-281 (P x P column-major complex); parity hook */
+int rbmgpu_build_S(rbmgpu_ctx* ctx, double r1, double r2, double rd, double* S);
+
 /* one S.x product (the Aprod lambda of cg_solver.cpp:301-317) */
 int rbmgpu_apply_S(rbmgpu_ctx* ctx, const double* x, double* y, double r1,
                    double r2, double rd);

@@ -55,6 +55,15 @@ int rbmhost_initial_weights(const rbmhost_model* m, uint64_t seed, double* param
  * finite, else 0 (the driver then exits with rc 55).  dw: n complex. */
 int rbmhost_step_ok(const double* dw, int64_t n, double* max_abs);
 
+/* update plugins of optimizer.plugin (plugin.cpp:30-77): kind "adam"
+ * (p1..p3 = beta1, beta2, eps) or "momentum" (p1 = alpha, p2 = dialup).
+ * apply turns the solver step dw (n complex, in place) into the update,
+ * learning rate included, exactly as base_plugin::apply. */
+typedef struct rbmhost_plugin rbmhost_plugin;
+rbmhost_plugin* rbmhost_plugin_create(const char* kind, int64_t n, double p1, double p2, double p3);
+int rbmhost_plugin_apply(rbmhost_plugin* pl, double* dw, int64_t n, double lr);
+void rbmhost_plugin_destroy(rbmhost_plugin* pl);
+
 /* GPU context for this model; precision / lanes / chains / samples of 0
  * keep the .ini values */
 int rbmhost_gpu_create(const rbmhost_model* m, int32_t precision, int32_t lanes, rbmgpu_ctx** out);

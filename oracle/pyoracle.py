@@ -339,3 +339,42 @@ def sigma_of(states, N):
     for i in range(N):
         out[:, i] = 1.0 - 2.0 * ((s[:, i // 64] >> np.uint64(i % 64)) & np.uint64(1)).astype(np.float64)
     return out
+
+
+# ---- optimizer plugins and SGD gradient (numpy restatement; O(P) vector
+# arithmetic, so numpy is the oracle here)
+def adam_steps(dws, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """adam_plugin::apply (plugin.cpp:43-57) over a sequence of solver steps"""
+    n = len(dws[0])
+    m = np.zeros(n, np.complex128)
+    wr = np.zeros(n)
+    wi = np.zeros(n)
+    out = []
+    for t, dw in enumerate(dws, start=1):
+        dw = np.asarray(dw, np.complex128)
+        m = beta1 * m + (1 - beta1) * dw
+        wr = beta2 * wr + (1 - beta2) * dw.real ** 2
+        wi = beta2 * wi + (1 - beta2) * dw.imag ** 2
+        mh = m / (1 - beta1 ** t)
+        wrh = wr / (1 - beta2 ** t)
+        wih = wi / (1 - beta2 ** t)
+        out.append(lr * (mh.real / (np.sqrt(wrh) + eps) + 1j * mh.imag / (np.sqrt(wih) + eps)))
+    return out
+
+
+def momentum_steps(dws, lr, alpha=0.9, dialup=1.0):
+    """momentum_plugin::apply (plugin.cpp:69-77) over a sequence of steps"""
+    m = np.zeros(len(dws[0]), np.complex128)
+    dup = 1.0 if dialup == 1.0 else 1e-3
+    out = []
+    for dw in dws:
+        dw = np.asarray(dw, np.complex128)
+        ax = alpha
+        if np.all(np.abs(m) <= 1e-12):  # Eigen isZero
+            m = dw.copy()
+        if dup < 1.0:
+            dup *= dialup
+            ax *= dup
+        m = ax * m + (1 - ax) * dw
+        out.append(lr * m)
+    return out

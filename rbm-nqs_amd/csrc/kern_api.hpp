@@ -57,6 +57,7 @@ struct SampArgs {
     int warmup, step, npb;
     uint64_t thr;  // bond flip if philox word < thr (thr = bond_flips * 2^32)
     uint32_t epoch, k0, k1;
+    const uint32_t* epoch_dev;  // non-null: the epoch is read from device memory (graph replays)
     int pf_refresh;  // fresh Pfaffian inverse every pf_refresh accepted moves (0: never)
     int obs_later;   // 1: kept samples store state and tanh only, k_eloc adds the observables
     const int32_t* pb;
@@ -146,6 +147,7 @@ struct SxArgs {
     const double* var;   // P
     const double* maxvar;
     double r1, r2, rd;
+    const double* reg_dev;  // non-null: r1, r2, rd read from device memory (graph replays)
     cplx<R>* part;       // nblocks x ld
     double* part2;       // nchunks x P complex
     double* y;           // complex P
@@ -236,6 +238,29 @@ hipError_t launch_sx_reduce(const SxArgs<R>& a, hipStream_t st);
 template <typename R>
 hipError_t launch_sx_colsum(const SxArgs<R>& a, hipStream_t st);
 constexpr int kColBlock = 64;  // columns per k_sx_colsum block
+
+// explicit S (kern_dense.hip): G = X^T X of the O matrix viewed as S x ncol
+// reals (ncol = 2 ld), 64 x 64 upper-triangle tiles, ksplit sample slices
+// of kper samples (a multiple of 32) each writing its own partial tiles
+template <typename R>
+struct SyrkArgs {
+    const R* X;
+    int64_t S, kper;
+    int ncol, nvalid, ntile, npairs, ksplit;  // nvalid = 2 P: columns past it read as 0
+    R* part;                                  // ksplit x npairs x 64 x 64
+};
+template <typename R>
+hipError_t launch_syrk(const SyrkArgs<R>& a, hipStream_t st);
+// S (P x P complex, column-major) from the partials: mode 0 the CG
+// regularisation, mode 1 the direct solver's
+template <typename R>
+hipError_t launch_s_from_g(const SyrkArgs<R>& a, int P, int n_neural, int64_t S, const double* mean,
+                           const double* var, const double* maxvar, double r1, double r2, double rd, int mode,
+                           double* Sm, hipStream_t st);
+// y = S x of the Hermitian S, per-block partials of x^H y in pdot
+// (ceil(P / 4) blocks); returns at once once st->done
+hipError_t launch_dense_mv(const double* Sm, const double* x, double* y, int P, double* pdot, const CgStatus* st,
+                           hipStream_t s);
 
 // supported (lanes per chain, units per lane) shapes
 struct Shape {

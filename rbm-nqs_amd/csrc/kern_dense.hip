@@ -1,0 +1,256 @@
+// kern_dense.hip -- the explicit S of the cg-single and direct SR solvers
+// (cg_solver.cpp:274-299, direct_solver.cpp:31-59) and the dense S.x of
+// the CG on it.
+//
+// S_ij = (1/S) sum_k conj(O_ki) O_kj - conj(mu_i) mu_j (+ regularisation).
+// O is stored one sample per row (ld complex), i.e. as the real S x 2 ld
+// matrix X with X[k][2p] = Re O_kp, X[k][2p+1] = Im O_kp, so G = X^T X is a
+// real SYRK and
+//   Re S_ij ~ G[2i][2j] + G[2i+1][2j+1],  Im S_ij ~ G[2i][2j+1] - G[2i+1][2j].
+// k_syrk_*: one block per upper-triangle 64 x 64 tile of G and K slice
+// (sample range); 32-sample chunks of both panels staged in LDS; FP32 on
+// MFMA (each of the 4 waves owns a 32 x 32 quarter of the tile as 2 x 2
+// v_mfma_f32_16x16x4_f32 tiles), FP64 by VALU FMAs (4 x 4 per thread);
+// each K slice writes its own partial tile (no atomics, a fixed summation
+// order).  k_s_from_g sums the slices in FP64 and forms S.
+//
+// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator r of lane l is D[4(l/16)+r][l%16].
+#include <hip/hip_runtime.h>
+
+#include "kern_api.hpp"
+
+namespace rbmgpu {
+
+typedef float v4f_d __attribute__((ext_vector_type(4)));
+
+constexpr int kGT = 64;     // G tile edge
+constexpr int kGChunk = 32; // samples per LDS chunk
+
+// tile pair q of the upper triangle (ti <= tj), row-major over ti
+__device__ inline void pair_of(int q, int nt, int& ti, int& tj) {
+    ti = 0;
+    int rem = q;
+    while (rem >= nt - ti) {
+        rem -= nt - ti;
+        ti++;
+    }
+    tj = ti + rem;
+}
+
+// the 64 x 64 tile's panels for samples kc .. kc + 31 into LDS (zeros past
+// the slice end and past the valid columns)
+template <typename R>
+__device__ inline void syrk_stage(const SyrkArgs<R>& a, R (*As)[kGT + 1], R (*Bs)[kGT + 1], int c0, int c1,
+                                  int64_t kc, int64_t k0, int64_t k1) {
+#pragma unroll
+    for (int q = 0; q < kGChunk * kGT / 256; q++) {
+        const int e = threadIdx.x + 256 * q, s = e / kGT, c = e % kGT;
+        const int64_t k = kc + s;
+        const bool in = k < k1;
+        const int64_t row = (in ? k : k0) * a.ncol;
+        As[s][c] = (in && c0 + c < a.nvalid) ? a.X[row + c0 + c] : (R)0;
+        Bs[s][c] = (in && c1 + c < a.nvalid) ? a.X[row + c1 + c] : (R)0;
+    }
+}
+
+// FP32: MFMA, each wave a 32 x 32 quarter of the tile (2 x 2 MFMA tiles)
+__global__ void __launch_bounds__(256) k_syrk_f32(SyrkArgs<float> a) {
+    __shared__ float As[kGChunk][kGT + 1];
+    __shared__ float Bs[kGChunk][kGT + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
+    const int wi = w >> 1, wj = w & 1;
+    int ti, tj;
+    pair_of(blockIdx.x, a.ntile, ti, tj);
+    const int64_t k0 = (int64_t)blockIdx.y * a.kper, k1 = k0 + a.kper < a.S ? k0 + a.kper : a.S;
+    v4f_d acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = v4f_d{0, 0, 0, 0};
+    for (int64_t kc = k0; kc < k1; kc += kGChunk) {
+        syrk_stage<float>(a, As, Bs, ti * kGT, tj * kGT, kc, k0, k1);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGChunk; kk += 4) {
+            const float a0 = As[kk + lq][wi * 32 + lr], a1 = As[kk + lq][wi * 32 + 16 + lr];
+            const float b0 = Bs[kk + lq][wj * 32 + lr], b1 = Bs[kk + lq][wj * 32 + 16 + lr];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float* out = a.part + ((int64_t)blockIdx.y * a.npairs + blockIdx.x) * (kGT * kGT);
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = wi * 32 + x * 16 + 4 * lq + r, j = wj * 32 + y * 16 + lr;
+                out[i * kGT + j] = acc[x][y][r];
+            }
+}
+
+// FP64: VALU FMAs (MI355X runs FP64 vector and matrix FMAs at the same
+// peak); thread t owns rows 4 (t / 16) .. +3 and columns t % 16 + 16 c
+__global__ void __launch_bounds__(256) k_syrk_f64(SyrkArgs<double> a) {
+    __shared__ double As[kGChunk][kGT + 1];
+    __shared__ double Bs[kGChunk][kGT + 1];
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    int ti, tj;
+    pair_of(blockIdx.x, a.ntile, ti, tj);
+    const int64_t k0 = (int64_t)blockIdx.y * a.kper, k1 = k0 + a.kper < a.S ? k0 + a.kper : a.S;
+    double acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = 0;
+    for (int64_t kc = k0; kc < k1; kc += kGChunk) {
+        syrk_stage<double>(a, As, Bs, ti * kGT, tj * kGT, kc, k0, k1);
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < kGChunk; k++) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) av[x] = As[k][4 * ty + x];
+#pragma unroll
+            for (int y = 0; y < 4; y++) bv[y] = Bs[k][tx + 16 * y];
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) acc[x][y] = fma(av[x], bv[y], acc[x][y]);
+        }
+        __syncthreads();
+    }
+    double* out = a.part + ((int64_t)blockIdx.y * a.npairs + blockIdx.x) * (kGT * kGT);
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) out[(4 * ty + x) * kGT + tx + 16 * y] = acc[x][y];
+}
+
+// G[u][v] summed over the K slices in FP64 (u, v < 2 ld; symmetric)
+template <typename R>
+__device__ inline double g_at(const SyrkArgs<R>& a, int u, int v) {
+    int tu = u / kGT, tv = v / kGT;
+    if (tu > tv) {
+        const int t = u;
+        u = v;
+        v = t;
+        const int tt = tu;
+        tu = tv;
+        tv = tt;
+    }
+    const int q = tu * a.ntile - tu * (tu - 1) / 2 + (tv - tu);
+    const R* p = a.part + (int64_t)q * (kGT * kGT) + (u % kGT) * kGT + (v % kGT);
+    double s = 0;
+    for (int k = 0; k < a.ksplit; k++) s += (double)p[(int64_t)k * a.npairs * (kGT * kGT)];
+    return s;
+}
+
+// S from G: thread (i, j) of the P x P complex matrix (column-major),
+// mode 0: + r1 Var(O) (+ rd on the Pfaffian block) + r2 max Var on the
+// diagonal (cg_solver.cpp:276-281); mode 1: the diagonal scaled by 1 + r1
+// (1 + r1 + rd on the Pfaffian block), then + r2 max Var (direct_solver.cpp:45-53)
+template <typename R>
+__global__ void __launch_bounds__(256) k_s_from_g(SyrkArgs<R> a, int P, int n_neural, int64_t S, const double* mean,
+                                                  const double* var, const double* maxvar, double r1, double r2,
+                                                  double rd, int mode, double* Sm) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)P * P) return;
+    const int i = (int)(t % P), j = (int)(t / P);
+    const double inv = 1.0 / (double)S;
+    double re = (g_at(a, 2 * i, 2 * j) + g_at(a, 2 * i + 1, 2 * j + 1)) * inv;
+    double im = (g_at(a, 2 * i, 2 * j + 1) - g_at(a, 2 * i + 1, 2 * j)) * inv;
+    const double mri = mean[2 * i], mii = mean[2 * i + 1], mrj = mean[2 * j], mij = mean[2 * j + 1];
+    re -= mri * mrj + mii * mij;  // conj(mu_i) mu_j
+    im -= mri * mij - mii * mrj;
+    if (i == j) {
+        const double f = i < n_neural ? r1 : r1 + rd;
+        if (mode == 0) re += f * var[i] + r2 * (*maxvar);
+        else re = re * (1.0 + f) + r2 * (*maxvar);
+        if (mode == 1) im *= 1.0 + f;
+    }
+    Sm[2 * t] = re;
+    Sm[2 * t + 1] = im;
+}
+
+// y = S x for the Hermitian S (column-major): y_i = sum_j conj(S_ji) x_j, so
+// wave i reads column i contiguously; per block the partial x^H y of its
+// rows for the CG step.  4 rows per block.
+__global__ void __launch_bounds__(256) k_dense_mv(const double* Sm, const double* x, double* y, int P, double* pdot,
+                                                  const CgStatus* st) {
+    if (st && st->done) return;
+    __shared__ double red[2][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = blockIdx.x * 4 + w;
+    double yr = 0, yi = 0;
+    if (i < P) {
+        const double* col = Sm + 2 * (int64_t)i * P;
+        for (int j = lane; j < P; j += 64) {
+            const double sr = col[2 * j], si = -col[2 * j + 1], xr = x[2 * j], xi = x[2 * j + 1];
+            yr += sr * xr - si * xi;
+            yi += sr * xi + si * xr;
+        }
+    }
+    yr = group_sum<64>(yr);
+    yi = group_sum<64>(yi);
+    double dr = 0, di = 0;
+    if (i < P) {
+        if (lane == 0) {
+            y[2 * i] = yr;
+            y[2 * i + 1] = yi;
+        }
+        const double xr = x[2 * i], xi = x[2 * i + 1];
+        dr = xr * yr + xi * yi;  // conj(x_i) y_i
+        di = xr * yi - xi * yr;
+    }
+    if (lane == 0) {
+        red[0][w] = dr;
+        red[1][w] = di;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && pdot) {
+        pdot[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        pdot[2 * blockIdx.x + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+template <>
+hipError_t launch_syrk<float>(const SyrkArgs<float>& a, hipStream_t st) {
+    if (a.ntile < 1 || a.ksplit < 1 || a.kper % kGChunk) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_syrk_f32, dim3((unsigned)a.npairs, (unsigned)a.ksplit), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+template <>
+hipError_t launch_syrk<double>(const SyrkArgs<double>& a, hipStream_t st) {
+    if (a.ntile < 1 || a.ksplit < 1 || a.kper % kGChunk) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)a.npairs, (unsigned)a.ksplit), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_s_from_g(const SyrkArgs<R>& a, int P, int n_neural, int64_t S, const double* mean,
+                           const double* var, const double* maxvar, double r1, double r2, double rd, int mode,
+                           double* Sm, hipStream_t st) {
+    const int64_t n = (int64_t)P * P;
+    hipLaunchKernelGGL(k_s_from_g<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, P, n_neural, S, mean,
+                       var, maxvar, r1, r2, rd, mode, Sm);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_mv(const double* Sm, const double* x, double* y, int P, double* pdot, const CgStatus* st,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_dense_mv, dim3((unsigned)((P + 3) / 4)), dim3(256), 0, s, Sm, x, y, P, pdot, st);
+    return hipGetLastError();
+}
+
+#define DENSE_INST(R)                                                                                           \
+    template hipError_t launch_s_from_g<R>(const SyrkArgs<R>&, int, int, int64_t, const double*, const double*, \
+                                           const double*, double, double, double, int, double*, hipStream_t);
+DENSE_INST(float)
+DENSE_INST(double)
+
+}  // namespace rbmgpu

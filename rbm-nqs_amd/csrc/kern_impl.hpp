@@ -631,10 +631,11 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
     int64_t nc = spc + (chain == 0 ? res : 0);
     int64_t off = chain == 0 ? 0 : chain * spc + res;
     int64_t total = nc * a.step + a.warmup;
+    const uint32_t epoch = a.epoch_dev ? *a.epoch_dev : a.epoch;
 
     uint64_t st[2] = {0, 0};
     {
-        u4 r = philox((uint32_t)chain, a.epoch, 0xFFFFFFFFu, 0u, a.k0, a.k1);
+        u4 r = philox((uint32_t)chain, epoch, 0xFFFFFFFFu, 0u, a.k0, a.k1);
         uint32_t w[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
         for (int q = 0; q < 4; q++) st[q >> 1] |= (uint64_t)w[q] << (32 * (q & 1));
@@ -657,7 +658,7 @@ __global__ void __launch_bounds__(256) k_sample(SampArgs<R> a) {
     int pcode = 0;
     R pu = 0;
     auto draw = [&](int64_t step, int sw0) {
-        u4 r = philox((uint32_t)chain, a.epoch, (uint32_t)step, (uint32_t)(sw0 + ln), a.k0, a.k1);
+        u4 r = philox((uint32_t)chain, epoch, (uint32_t)step, (uint32_t)(sw0 + ln), a.k0, a.k1);
         int f0, f1 = -1;
         if ((uint64_t)r.x < a.thr && d.npb > 0) {
             int bi = (int)bounded(r.y, (uint32_t)d.npb);
@@ -1056,7 +1057,9 @@ __global__ void __launch_bounds__(1024) k_sx_colsum(SxArgs<R> a) {
             zi += zs[q][lane][1];
         }
         double inv = 1.0 / (double)a.S;
-        double reg = a.var[c] * (c < a.n_neural ? a.r1 : a.r1 + a.rd) + (*a.maxvar) * a.r2;
+        const double r1 = a.reg_dev ? a.reg_dev[0] : a.r1, r2 = a.reg_dev ? a.reg_dev[1] : a.r2,
+                     rd = a.reg_dev ? a.reg_dev[2] : a.rd;
+        double reg = a.var[c] * (c < a.n_neural ? r1 : r1 + rd) + (*a.maxvar) * r2;
         double xr = a.x[2 * c], xi = a.x[2 * c + 1];
         double ur = a.mean[2 * c], ui = -a.mean[2 * c + 1];  // conj(mu_c)
         double yr = zr * inv + reg * xr - (ur * mr - ui * mi);

@@ -160,9 +160,10 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
     // every chain of the block runs the block's longest schedule (lockstep)
     int64_t total = (spc + (blockIdx.x == 0 ? res : 0)) * a.step + a.warmup;
 
+    const uint32_t epoch = a.epoch_dev ? *a.epoch_dev : a.epoch;
     uint64_t st[2] = {0, 0};
     {
-        u4 r = philox((uint32_t)chain, a.epoch, 0xFFFFFFFFu, 0u, a.k0, a.k1);
+        u4 r = philox((uint32_t)chain, epoch, 0xFFFFFFFFu, 0u, a.k0, a.k1);
         uint32_t wd[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
         for (int q = 0; q < 4; q++) st[q >> 1] |= (uint64_t)wd[q] << (32 * (q & 1));
@@ -186,8 +187,8 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
     float pua = 0, pus = 0;
     auto draw = [&](int64_t step, int sw0) {
         uint32_t slot = (uint32_t)(sw0 + lane);
-        u4 A = philox((uint32_t)chain, a.epoch, (uint32_t)step, slot, a.k0, a.k1);
-        u4 B = philox((uint32_t)chain, a.epoch, (uint32_t)step, slot | 0x80000000u, a.k0, a.k1);
+        u4 A = philox((uint32_t)chain, epoch, (uint32_t)step, slot, a.k0, a.k1);
+        u4 B = philox((uint32_t)chain, epoch, (uint32_t)step, slot | 0x80000000u, a.k0, a.k1);
         if (A.x < la.lam_thr) {
             pcode = (int)(0x40000000u | ((1u + bounded(B.y, (uint32_t)kmax)) << 16));
         } else {

@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <complex>
 #include <cstring>
 #include <random>
@@ -86,6 +88,24 @@ struct rbmgpu_ctx {
     hipEvent_t ev[14] = {};
     double times[6] = {0, 0, 0, 0, 0, 0};
     int64_t cg_last_itn = 0;  // iterations of the last CG solve
+    // ---- device-resident SR epoch (rbmgpu_train_epoch)
+    double* d_params = nullptr;        // FP64 parameters on the device
+    bool params_dev = false;           // d_params is newer than params
+    void* d_par = nullptr;             // DevPar
+    void* h_par = nullptr;             // pinned host mirror
+    void* d_stats = nullptr;           // DevStats
+    void* h_stats = nullptr;           // pinned host mirror
+    const double* reg_dev = nullptr;   // set while an epoch body is enqueued
+    const uint32_t* epoch_dev = nullptr;
+    bool capturing = false;
+    uint64_t gen = 0;                  // bumped by every device (re)allocation
+    uint64_t graph_gen = 0;
+    std::vector<std::pair<int64_t, hipGraphExec_t>> graphs;  // CG budget -> instantiated epoch
+    int64_t train_itn = 0;             // CG iterations of the last device epoch
+    // ---- explicit S (cg-single / direct)
+    double* d_Smat = nullptr;  // P x P complex, column-major
+    void* d_gpart = nullptr;   // SYRK partial tiles
+    int64_t gpart_cap = 0;
     std::vector<char> staging;
     // ---- learned proposal
     int Hn = 0, Np = 0, kmax = 0;
@@ -132,6 +152,8 @@ int fail(rbmgpu_ctx* ctx, const std::string& m) {
 
 template <typename T>
 int dalloc(rbmgpu_ctx* ctx, T** p, size_t bytes) {
+    ctx->gen++;  // captured graphs hold the old addresses
+    if (ctx->capturing) return fail(ctx, "device allocation during graph capture");
     if (*p) {
         (void)hipFree(*p);
         *p = nullptr;
@@ -313,6 +335,7 @@ template <typename R>
 int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
     rbmgpu_ctx* ctx = c;
     SampArgs<R> a = samp_args<R>(c, epoch, block, n_samples, warmup);
+    a.epoch_dev = c->epoch_dev;
     a.obs_later = eloc_after(c);
     HIPCK(launch_sample<R>(a, s, block, c->stream));
     return 0;
@@ -564,6 +587,28 @@ int autotune(rbmgpu_ctx* c) {
     return 0;
 }
 
+// zero fill by a kernel: unlike hipMemsetAsync it may be captured into a
+// HIP graph and replayed
+__global__ void __launch_bounds__(256) k_zero(uint32_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+int zero_async(rbmgpu_ctx* c, void* p, size_t bytes) {
+    rbmgpu_ctx* ctx = c;
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n == 0) return 0;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4 * (int64_t)c->n_cu);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, c->stream, (uint32_t*)p, n);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+// timing events are left out of captured graphs
+int record(rbmgpu_ctx* c, int e) {
+    rbmgpu_ctx* ctx = c;
+    if (!c->capturing) HIPCK(hipEventRecord(c->ev[e], c->stream));
+    return 0;
+}
+
 float elapsed(rbmgpu_ctx* c, int a, int b) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0;
@@ -732,8 +777,31 @@ __global__ void __launch_bounds__(1024) k_e_final(const double* part, int64_t nb
     }
 }
 
+// parameters of a device-resident SR epoch, written by the host before each
+// graph replay (abstract_optimizer.cpp:80-178 with the decay of
+// abstract_optimizer.cpp:39-48 applied on the host)
+struct DevPar {
+    uint32_t epoch, pad;
+    int64_t max_iter;
+    double reg[3];  // r1, r2, r1 delta (read by k_sx_colsum)
+    double lr, rtol;
+};
+// results of a device-resident SR epoch
+struct DevStats {
+    double e[6];  // k_e_final: mean re/im, var, var binned, tau, std
+    double acc;   // mean acceptance over the chains
+    double rs, max_dw;
+    int64_t itn;
+    int32_t ok, pending;
+};
+
 __global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double* Ax, double* r, double* p,
-                                                  int P, CgStatus* st, int64_t max_iter, double rtol) {
+                                                  int P, CgStatus* st, int64_t max_iter, double rtol,
+                                                  const DevPar* par) {
+    if (par) {
+        max_iter = par->max_iter;
+        rtol = par->rtol;
+    }
     double s = 0, dummy = 0;
     for (int i = threadIdx.x; i < P; i += blockDim.x) {
         double rr = b[2 * i] - Ax[2 * i], ri = b[2 * i + 1] - Ax[2 * i + 1];
@@ -1016,7 +1084,7 @@ int struct_moments(rbmgpu_ctx* c) {
     a.P = (int)c->P;
     a.part2 = B;
     HIPCK(launch_sx_reduce<float>(a, c->stream));
-    HIPCK(hipMemsetAsync(c->d_maxvar, 0, sizeof(double), c->stream));
+    if (int rc = zero_async(c, c->d_maxvar, sizeof(double))) return rc;
     hipLaunchKernelGGL(k_struct_moments, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, A, B, nch,
                        c->S, c->N, c->na, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
                        (unsigned long long*)c->d_maxvar);
@@ -1054,7 +1122,9 @@ int ensure_cg(rbmgpu_ctx* c) {
     if (int rc = dalloc(ctx, &c->d_r, vb)) return rc;
     if (int rc = dalloc(ctx, &c->d_p, vb)) return rc;
     if (int rc = dalloc(ctx, &c->d_Ap, vb)) return rc;
-    if (int rc = dalloc(ctx, &c->d_pdot, ((c->P + kColBlock - 1) / kColBlock) * 2 * sizeof(double))) return rc;
+    // partials of p^H Ap: one per S.x column block or per dense-S.x block of 4 rows
+    const int64_t npd = std::max<int64_t>((c->P + kColBlock - 1) / kColBlock, (c->P + 3) / 4);
+    if (int rc = dalloc(ctx, &c->d_pdot, npd * 2 * sizeof(double))) return rc;
     if (int rc = dalloc(ctx, &c->d_st, sizeof(CgStatus))) return rc;
     return 0;
 }
@@ -1103,6 +1173,7 @@ template <typename R>
 SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r2, double rd,
                   const CgStatus* st) {
     SxArgs<R> a{};
+    a.reg_dev = c->reg_dev;
     a.S = c->S;
     a.nblocks = c->sx_blocks;
     a.P = (int)c->P;
@@ -1209,7 +1280,7 @@ int compute_moments(rbmgpu_ctx* c) {
         if (int rc = dalloc(ctx, &c->d_epart, pneed)) return rc;
         c->epart_cap = pneed;
     }
-    HIPCK(hipEventRecord(c->ev[4], c->stream));
+    if (int rc = record(c, 4)) return rc;
     hipLaunchKernelGGL(k_e_part, dim3((unsigned)nblk), dim3(256), 0, c->stream, c->d_obs, c->S, c->n_groups, chunk,
                        c->d_epart);
     HIPCK(hipGetLastError());
@@ -1240,13 +1311,13 @@ int compute_moments(rbmgpu_ctx* c) {
                               c->d_part};
             HIPCK(launch_o_partial<double>(a, (int)blocks, c->stream));
         }
-        HIPCK(hipMemsetAsync(c->d_maxvar, 0, sizeof(double), c->stream));
+        if (int rc = zero_async(c, c->d_maxvar, sizeof(double))) return rc;
         hipLaunchKernelGGL(k_o_final, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->d_part,
                            blocks, chunk, c->S, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
                            (unsigned long long*)c->d_maxvar);
         HIPCK(hipGetLastError());
     }
-    HIPCK(hipEventRecord(c->ev[5], c->stream));
+    if (int rc = record(c, 5)) return rc;
     c->moments_valid = true;
     return 0;
 }
@@ -1298,6 +1369,360 @@ int eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi, dou
          double* ratio, const int32_t* flips, int nf) {
     return c->prec == RBMGPU_FP32 ? run_eval<float>(c, d_states, n, logpsi, obs, Tout, ratio, flips, nf)
                                   : run_eval<double>(c, d_states, n, logpsi, obs, Tout, ratio, flips, nf);
+}
+
+// ---- device-resident SR epoch (rbmgpu_train_epoch): sample -> moments ->
+// F (SR.cpp:102) -> CG (cg_solver.cpp:119-189) -> massive-step guard and
+// w -= lr x (abstract_optimizer.cpp:153-172, rbm_base.cpp:117-143) ->
+// weight tables, with every per-epoch value (epoch, lr, regularisation,
+// iteration cap) read from DevPar, so the whole body can be captured once
+// in a HIP graph and replayed.
+
+// F = <E O*> - <E> conj(<O>)  (SR.cpp:102)
+__global__ void __launch_bounds__(256) k_sr_grad(const double* emom, const double* eo, const double* om, int P,
+                                                 double* b) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    {
+#pragma clang fp contract(off)  // each product rounded on its own, as the host computes F
+        const double er = emom[0], ei = emom[1], mr = om[2 * p], mi = om[2 * p + 1];
+        b[2 * p] = eo[2 * p] - (er * mr + ei * mi);  // E conj(mu) = (er + i ei)(mr - i mi)
+        b[2 * p + 1] = eo[2 * p + 1] - (ei * mr - er * mi);
+    }
+}
+
+// after a finished CG solve: the guard max |x| <= 30 with every entry
+// finite, then x <- lr x (the next epoch's warm start, as the reference
+// scales dw_ in place) and w -= x.  One block.  A solve that has not
+// finished leaves everything as it is and flags `pending`.
+__global__ void __launch_bounds__(1024) k_sr_update(const DevPar* par, const CgStatus* st, double* x,
+                                                    double* params, int P, DevStats* out) {
+    __shared__ double smx[16];
+    __shared__ int sbad[16];
+    if (!st->done) {
+        if (threadIdx.x == 0) {
+            out->pending = 1;
+            out->ok = 0;
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    double mx = 0;
+    int bad = 0;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const double a = hypot(x[2 * i], x[2 * i + 1]);
+        if (!isfinite(a)) bad = 1;
+        else mx = fmax(mx, a);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        bad |= __shfl_xor(bad, o);
+    }
+    if (lane == 0) {
+        smx[w] = mx;
+        sbad[w] = bad;
+    }
+    __syncthreads();
+    mx = 0;
+    bad = 0;
+    for (int q = 0; q < nw; q++) {
+        mx = fmax(mx, smx[q]);
+        bad |= sbad[q];
+    }
+    const bool ok = !bad && mx <= 30.0;
+    if (ok) {
+#pragma clang fp contract(off)  // w - (lr x) with lr x rounded first, as on the host
+        const double lr = par->lr;
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            const double xr = lr * x[2 * i], xi = lr * x[2 * i + 1];
+            x[2 * i] = xr;
+            x[2 * i + 1] = xi;
+            params[2 * i] -= xr;
+            params[2 * i + 1] -= xi;
+        }
+    }
+    if (threadIdx.x == 0) {
+        out->ok = ok;
+        out->pending = 0;
+        out->max_dw = bad ? (double)NAN : mx;
+        out->itn = st->itn;
+        out->rs = st->rs;
+    }
+}
+
+// weight tables from the device parameters (the layout of
+// build_weight_tables), after an accepted update only
+template <typename R>
+__global__ void __launch_bounds__(256) k_tables(const double* params, int N, int na, int nvb, int nsymm,
+                                                const int32_t* vbsite, int pf_rows, int64_t P_rbm,
+                                                const DevStats* st, cplx<R>* W, cplx<R>* C, cplx<R>* Sh,
+                                                cplx<R>* hb, cplx<R>* rb, cplx<R>* lb, cplx<R>* fs) {
+    if (!st->ok) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nwt = (int64_t)N * na;
+    if (t < nwt) {
+        const int k = (int)(t / na), a = (int)(t % na);
+        const int64_t q = nvb + na + k + (int64_t)a * N;
+        const double x = 2 * params[2 * q], y = 2 * params[2 * q + 1];
+        double sy, cy;
+        sincos(y, &sy, &cy);
+        const double chx = cosh(x), shx = sinh(x);
+        W[t] = {(R)params[2 * q], (R)params[2 * q + 1]};
+        C[t] = {(R)(chx * cy), (R)(shx * sy)};   // cosh(x + iy)
+        Sh[t] = {(R)(shx * cy), (R)(chx * sy)};  // sinh(x + iy)
+    } else if (t < nwt + na) {
+        const int a = (int)(t - nwt);
+        hb[a] = {(R)params[2 * (nvb + a)], (R)params[2 * (nvb + a) + 1]};
+    } else if (t < nwt + na + N) {
+        const int f = (int)(t - nwt - na);
+        rb[f] = {(R)params[2 * (f % nvb)], (R)params[2 * (f % nvb) + 1]};
+        double lr = 0, li = 0;  // lb[f]: sum of v_i over the symmetries that map basis i to f
+        for (int s = 0; s < nsymm; s++)
+            for (int i = 0; i < nvb; i++)
+                if (vbsite[s * nvb + i] == f) {
+                    lr += params[2 * i];
+                    li += params[2 * i + 1];
+                }
+        lb[f] = {(R)lr, (R)li};
+    } else if (t < nwt + na + N + 4 * (int64_t)pf_rows) {
+        const int e = (int)(t - nwt - na - N), r = e / 4, q = e % 4;
+        const int64_t pi = P_rbm + r + (int64_t)q * pf_rows;
+        fs[e] = {(R)params[2 * pi], (R)params[2 * pi + 1]};
+    }
+}
+
+// energy statistics and the mean chain acceptance of the epoch
+__global__ void __launch_bounds__(1024) k_epoch_stats(const double* emom, const uint32_t* acc,
+                                                      const uint32_t* prop, int n_chains, DevStats* out) {
+    double a = 0, dummy = 0;
+    for (int i = threadIdx.x; i < n_chains; i += blockDim.x) a += prop[i] ? (double)acc[i] / prop[i] : 0.0;
+    block_sum2(a, dummy);
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 6; q++) out->e[q] = emom[q];
+        out->acc = n_chains > 0 ? a / n_chains : 0.0;
+    }
+}
+
+// explicit S of the current sample set (kern_dense.hip) into d_Smat: mode 0
+// with the CG regularisation (cg_solver.cpp:276-281), mode 1 with the
+// direct solver's (direct_solver.cpp:45-53)
+template <typename R>
+int build_S_t(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
+    rbmgpu_ctx* ctx = c;
+    SyrkArgs<R> a{};
+    a.X = (const R*)c->d_O;
+    a.S = c->S;
+    a.ncol = (int)(2 * c->ld);
+    a.nvalid = (int)(2 * c->P);
+    a.ntile = (int)((2 * c->P + 63) / 64);
+    a.npairs = a.ntile * (a.ntile + 1) / 2;
+    // about 4 blocks per CU over tiles x sample slices of >= 256 samples
+    int64_t ks = std::max<int64_t>(1, (4 * (int64_t)c->n_cu + a.npairs - 1) / a.npairs);
+    ks = std::min<int64_t>(ks, std::max<int64_t>(1, c->S / 256));
+    a.kper = ((c->S + ks - 1) / ks + 31) / 32 * 32;
+    a.ksplit = (int)((c->S + a.kper - 1) / a.kper);
+    const int64_t need = (int64_t)a.ksplit * a.npairs * 64 * 64 * (int64_t)sizeof(R);
+    if (need > c->gpart_cap) {
+        if (int rc = dalloc(ctx, &c->d_gpart, need)) return rc;
+        c->gpart_cap = need;
+    }
+    a.part = (R*)c->d_gpart;
+    HIPCK(launch_syrk<R>(a, c->stream));
+    HIPCK(launch_s_from_g<R>(a, (int)c->P, (int)c->P_rbm, c->S, c->d_omean, c->d_ovar, c->d_maxvar, r1, r2, rd, mode,
+                             c->d_Smat, c->stream));
+    return 0;
+}
+
+int build_S(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
+    rbmgpu_ctx* ctx = c;
+    if (c->structured || !c->d_O)
+        return fail(c, "explicit S needs the O matrix: set keep_o (gpu.keep_o = true) for the cg-single / direct "
+                       "solvers");
+    if (c->S < 1) return fail(c, "no samples");
+    if (!c->moments_valid)
+        if (int rc = compute_moments(c)) return rc;
+    if (!c->d_Smat)
+        if (int rc = dalloc(ctx, &c->d_Smat, c->P * c->P * 16)) return rc;
+    return c->prec == RBMGPU_FP32 ? build_S_t<float>(c, r1, r2, rd, mode) : build_S_t<double>(c, r1, r2, rd, mode);
+}
+
+// x = A^-1 b for the dense Hermitian A (column-major) on the host: Cholesky
+// A = L L^H, or LU with partial pivoting when A is not positive definite.
+// The direct solver's regularised S is positive definite for r2 > 0, where
+// the reference's complete orthogonal decomposition gives the same x.
+int dense_solve(rbmgpu_ctx* c, const std::vector<zd>& A, const std::vector<zd>& b, std::vector<zd>& x) {
+    const int64_t n = (int64_t)b.size();
+    std::vector<zd> L(A);
+    bool pd = true;
+    for (int64_t j = 0; j < n && pd; j++) {
+        double d = L[j + j * n].real();
+        for (int64_t k = 0; k < j; k++) d -= std::norm(L[j + k * n]);
+        if (!(d > 0)) {
+            pd = false;
+            break;
+        }
+        d = std::sqrt(d);
+        L[j + j * n] = d;
+        for (int64_t i = j + 1; i < n; i++) {
+            zd s = L[i + j * n];
+            for (int64_t k = 0; k < j; k++) s -= L[i + k * n] * std::conj(L[j + k * n]);
+            L[i + j * n] = s / d;
+        }
+    }
+    if (pd) {
+        std::vector<zd> y(n);
+        for (int64_t i = 0; i < n; i++) {
+            zd s = b[i];
+            for (int64_t k = 0; k < i; k++) s -= L[i + k * n] * y[k];
+            y[i] = s / L[i + i * n].real();
+        }
+        for (int64_t i = n - 1; i >= 0; i--) {
+            zd s = y[i];
+            for (int64_t k = i + 1; k < n; k++) s -= std::conj(L[k + i * n]) * x[k];
+            x[i] = s / L[i + i * n].real();
+        }
+        return 0;
+    }
+    std::vector<zd> M(A), r(b);
+    std::vector<int64_t> piv(n);
+    for (int64_t j = 0; j < n; j++) {
+        int64_t p = j;
+        for (int64_t i = j + 1; i < n; i++)
+            if (std::abs(M[i + j * n]) > std::abs(M[p + j * n])) p = i;
+        if (std::abs(M[p + j * n]) == 0) return fail(c, "direct solver: singular S");
+        if (p != j) {
+            for (int64_t k = 0; k < n; k++) std::swap(M[j + k * n], M[p + k * n]);
+            std::swap(r[j], r[p]);
+        }
+        for (int64_t i = j + 1; i < n; i++) {
+            const zd f = M[i + j * n] / M[j + j * n];
+            if (f == zd(0)) continue;
+            for (int64_t k = j; k < n; k++) M[i + k * n] -= f * M[j + k * n];
+            r[i] -= f * r[j];
+        }
+    }
+    for (int64_t i = n - 1; i >= 0; i--) {
+        zd s = r[i];
+        for (int64_t k = i + 1; k < n; k++) s -= M[i + k * n] * x[k];
+        x[i] = s / M[i + i * n];
+    }
+    return 0;
+}
+
+// one sampling epoch on the stream: all chains, the kept samples' local
+// observables and (materialised path) the O matrix; buffers allocated
+int sample_body(rbmgpu_ctx* c, uint32_t epoch) {
+    rbmgpu_ctx* ctx = c;
+    if (int rc = zero_async(c, c->d_obs, c->n_samples * 2 * c->n_groups * sizeof(double))) return rc;
+    if (int rc = record(c, 0)) return rc;
+    int rc = 0;
+    if (c->Hn && c->lam > 0) {
+        LearnArgs la = learn_args(c, epoch);
+        la.s.epoch_dev = c->epoch_dev;
+        HIPCK(launch_sample_mlp(la, learn_k(c), c->stream));
+    } else {
+        rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, epoch, c->shape, c->block, c->n_samples, c->warmup)
+                                    : run_sample<double>(c, epoch, c->shape, c->block, c->n_samples, c->warmup);
+        if (!rc && eloc_after(c)) rc = run_eloc(c, c->n_samples);
+    }
+    if (rc) return rc;
+    if (int rc2 = record(c, 1)) return rc2;
+    c->learned_epoch = c->Hn && c->lam > 0;
+    c->S = c->n_samples;
+    c->have_chains = true;
+    rc = c->prec == RBMGPU_FP32 ? run_build_O<float>(c) : run_build_O<double>(c);
+    if (rc) return rc;
+    if (int rc2 = record(c, 2)) return rc2;
+    c->moments_valid = false;
+    return 0;
+}
+
+
+int ensure_train(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (!c->d_par) {
+        if (int rc = dalloc(ctx, &c->d_par, sizeof(DevPar))) return rc;
+        if (int rc = dalloc(ctx, &c->d_stats, sizeof(DevStats))) return rc;
+        HIPCK(hipHostMalloc(&c->h_par, sizeof(DevPar), hipHostMallocDefault));
+        HIPCK(hipHostMalloc(&c->h_stats, sizeof(DevStats), hipHostMallocDefault));
+        // the reference's dw_ starts at zero (SR.cpp:57)
+        if (int rc = zero_async(c, c->d_x, c->P * 2 * sizeof(double))) return rc;
+    }
+    return 0;
+}
+
+// guard, update, tables and statistics after the CG launches
+int train_tail(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    hipLaunchKernelGGL(k_sr_update, dim3(1), dim3(1024), 0, c->stream, (const DevPar*)c->d_par, c->d_st, c->d_x,
+                       c->d_params, (int)c->P, (DevStats*)c->d_stats);
+    HIPCK(hipGetLastError());
+    const int64_t nt = (int64_t)c->N * c->na + c->na + c->N + 4 * (int64_t)c->pf_rows;
+    if (c->prec == RBMGPU_FP32) {
+        Tabs<float> t = tabs<float>(c);
+        hipLaunchKernelGGL(k_tables<float>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, c->d_params,
+                           c->N, c->na, c->nvb, c->nsymm, c->d_vbsite, c->pf_rows, c->P_rbm,
+                           (const DevStats*)c->d_stats, (cplx<float>*)t.W, (cplx<float>*)t.C, (cplx<float>*)t.S,
+                           (cplx<float>*)t.hb, (cplx<float>*)t.rb, (cplx<float>*)t.lb, (cplx<float>*)t.fs);
+    } else {
+        Tabs<double> t = tabs<double>(c);
+        hipLaunchKernelGGL(k_tables<double>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_params, c->N, c->na, c->nvb, c->nsymm, c->d_vbsite, c->pf_rows, c->P_rbm,
+                           (const DevStats*)c->d_stats, (cplx<double>*)t.W, (cplx<double>*)t.C,
+                           (cplx<double>*)t.S, (cplx<double>*)t.hb, (cplx<double>*)t.rb, (cplx<double>*)t.lb,
+                           (cplx<double>*)t.fs);
+    }
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_epoch_stats, dim3(1), dim3(1024), 0, c->stream, c->d_emom, c->d_acc, c->d_prop, c->n_chains,
+                       (DevStats*)c->d_stats);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+// the per-epoch values come from DevPar while an epoch is enqueued
+struct DevArgsScope {
+    rbmgpu_ctx* c;
+    explicit DevArgsScope(rbmgpu_ctx* cc) : c(cc) {
+        c->epoch_dev = (const uint32_t*)((const char*)c->d_par + offsetof(DevPar, epoch));
+        c->reg_dev = (const double*)((const char*)c->d_par + offsetof(DevPar, reg));
+    }
+    ~DevArgsScope() {
+        c->epoch_dev = nullptr;
+        c->reg_dev = nullptr;
+    }
+};
+
+// the whole epoch with n_cg CG iterations queued (launches past convergence
+// return at once)
+int train_body(rbmgpu_ctx* c, int64_t n_cg) {
+    rbmgpu_ctx* ctx = c;
+    DevArgsScope scope(c);
+    if (int rc = sample_body(c, 0)) return rc;
+    if (int rc = ensure_sx(c)) return rc;  // S.x layout of this sample set (allocates only when it grows)
+    if (int rc = compute_moments(c)) return rc;
+    hipLaunchKernelGGL(k_sr_grad, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->d_emom, c->d_eo,
+                       c->d_omean, (int)c->P, c->d_b);
+    HIPCK(hipGetLastError());
+    if (int rc = sx(c, c->d_x, c->d_Ap, 0, 0, 0, nullptr)) return rc;
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1024), 0, c->stream, c->d_b, c->d_Ap, c->d_r, c->d_p, (int)c->P,
+                       c->d_st, (int64_t)0, 0.0, (const DevPar*)c->d_par);
+    HIPCK(hipGetLastError());
+    for (int64_t i = 0; i < n_cg; i++)
+        if (int rc = sx(c, c->d_p, c->d_Ap, 0, 0, 0, c->d_st, c->d_pdot, true)) return rc;
+    return train_tail(c);
+}
+
+// more CG iterations of an unfinished solve, then the tail again
+int train_more(rbmgpu_ctx* c, int64_t n_cg) {
+    DevArgsScope scope(c);
+    for (int64_t i = 0; i < n_cg; i++)
+        if (int rc = sx(c, c->d_p, c->d_Ap, 0, 0, 0, c->d_st, c->d_pdot, true)) return rc;
+    return train_tail(c);
+}
+
+void drop_graphs(rbmgpu_ctx* c) {
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
 }
 
 }  // namespace
@@ -1508,9 +1933,12 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,
                     c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,
                     c->d_rec_k,  c->d_rec_a,  c->d_lacc,   c->d_lprop, c->d_tr,     c->d_tst,    c->d_tseed,
-                    c->d_tw,     c->d_ratio_term, c->d_term_slot};
+                    c->d_tw,     c->d_ratio_term, c->d_term_slot, c->d_params, c->d_par, c->d_stats, c->d_Smat, c->d_gpart};
+    drop_graphs(c);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->h_par) (void)hipHostFree(c->h_par);
+    if (c->h_stats) (void)hipHostFree(c->h_stats);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1533,14 +1961,47 @@ int rbmgpu_launch_shape(const rbmgpu_ctx* c, int32_t* o) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// the FP64 parameters on the device follow the host copy
+int push_params(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (int rc = build_weight_tables(c)) return rc;
+    if (!c->d_params)
+        if (int rc = dalloc(ctx, &c->d_params, c->P * 2 * sizeof(double))) return rc;
+    HIPCK(hipStreamSynchronize(c->stream));
+    HIPCK(hipMemcpy(c->d_params, c->params.data(), c->P * 16, hipMemcpyHostToDevice));
+    c->params_dev = false;
+    return 0;
+}
+// the host copy from the device after device-side updates
+int pull_params(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (!c->params_dev) return 0;
+    HIPCK(hipStreamSynchronize(c->stream));
+    HIPCK(hipMemcpy(c->params.data(), c->d_params, c->P * 16, hipMemcpyDeviceToHost));
+    c->params_dev = false;
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int rbmgpu_set_params(rbmgpu_ctx* c, const double* p) {
     if (!c || !p) return -1;
     for (int64_t i = 0; i < c->P; i++) c->params[i] = zd(p[2 * i], p[2 * i + 1]);
-    return build_weight_tables(c);
+    return push_params(c);
 }
 
 int rbmgpu_get_params(const rbmgpu_ctx* c, double* p) {
     if (!c || !p) return -1;
+    if (c->params_dev) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess ||
+            hipMemcpy(p, c->d_params, c->P * 16, hipMemcpyDeviceToHost) != hipSuccess)
+            return -2;
+        return 0;
+    }
     for (int64_t i = 0; i < c->P; i++) {
         p[2 * i] = c->params[i].real();
         p[2 * i + 1] = c->params[i].imag();
@@ -1550,35 +2011,114 @@ int rbmgpu_get_params(const rbmgpu_ctx* c, double* p) {
 
 int rbmgpu_update_weights(rbmgpu_ctx* c, const double* dw) {
     if (!c || !dw) return -1;
+    if (int rc = pull_params(c)) return rc;
     for (int64_t i = 0; i < c->P; i++) c->params[i] -= zd(dw[2 * i], dw[2 * i + 1]);
-    return build_weight_tables(c);
+    return push_params(c);
+}
+
+int rbmgpu_train_epoch(rbmgpu_ctx* c, uint32_t epoch, const rbmgpu_sr_params* sp, rbmgpu_epoch_stats* out) {
+    if (!c || !sp) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (!c->d_params) return fail(c, "rbmgpu_train_epoch: no parameters (rbmgpu_set_params first)");
+    if (int rc = autotune(c)) return rc;
+    if (int rc = ensure_samples(c, c->n_samples)) return rc;
+    if (int rc = ensure_cg(c)) return rc;
+    if (int rc = ensure_train(c)) return rc;
+    // cg_solver.cpp:48-60: max_iter = min(2P, max_iter), 0 -> 2P; rtol 0 -> 1e-16
+    int64_t mi = std::min<int64_t>(2 * c->P, sp->max_iter);
+    if (mi <= 0) mi = 2 * c->P;
+    DevPar* hp = (DevPar*)c->h_par;
+    hp->epoch = epoch;
+    hp->pad = 0;
+    hp->max_iter = mi;
+    hp->reg[0] = sp->reg1;
+    hp->reg[1] = sp->reg2;
+    hp->reg[2] = sp->reg1_delta;
+    hp->lr = sp->lr;
+    hp->rtol = sp->rtol == 0. ? 1e-16 : sp->rtol;
+    HIPCK(hipMemcpyAsync(c->d_par, c->h_par, sizeof(DevPar), hipMemcpyHostToDevice, c->stream));
+    // CG launches captured: the last solve's count rounded up to 8 (launches
+    // past convergence return at once); an unfinished solve continues below
+    const int64_t budget = std::min<int64_t>(mi, std::max<int64_t>(8, (c->train_itn + 2 + 7) / 8 * 8));
+    auto t0 = std::chrono::steady_clock::now();
+    if (sp->graph) {
+        if (c->graph_gen != c->gen) drop_graphs(c);
+        hipGraphExec_t ex = nullptr;
+        for (auto& g : c->graphs)
+            if (g.first == budget) ex = g.second;
+        if (ex) {
+            HIPCK(hipGraphLaunch(ex, c->stream));
+        } else {
+            // this epoch runs eagerly (allocating what it needs), then the
+            // same body is captured for the following epochs
+            if (int rc = train_body(c, budget)) return rc;
+            HIPCK(hipStreamSynchronize(c->stream));
+            if (c->graph_gen != c->gen) drop_graphs(c);
+            const uint64_t gen0 = c->gen;
+            hipGraph_t g = nullptr;
+            c->capturing = true;
+            hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+            int rc = e == hipSuccess ? train_body(c, budget) : 0;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            c->capturing = false;
+            if (e != hipSuccess) return fail(c, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e2 != hipSuccess) return fail(c, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+            if (c->gen != gen0) {
+                (void)hipGraphDestroy(g);
+                return fail(c, "device buffers changed during graph capture");
+            }
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(c, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            c->graphs.push_back({budget, ex});
+            c->graph_gen = c->gen;
+        }
+    } else {
+        if (int rc = train_body(c, budget)) return rc;
+    }
+    DevStats* hs = (DevStats*)c->h_stats;
+    HIPCK(hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    while (hs->pending) {
+        if (int rc = train_more(c, 8)) return rc;
+        HIPCK(hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->train_itn = hs->itn;
+    c->cg_last_itn = hs->itn;
+    c->moments_valid = true;
+    if (hs->ok) c->params_dev = true;
+    if (out) {
+        out->e_mean[0] = hs->e[0];
+        out->e_mean[1] = hs->e[1];
+        out->e_var = hs->e[2];
+        out->e_var_binned = hs->e[3];
+        out->e_tau = hs->e[4];
+        out->e_std = hs->e[5];
+        int64_t nb = 50;
+        while (c->S % nb != 0) nb--;
+        out->n_bins = nb;
+        out->acceptance = hs->acc;
+        out->cg_itn = hs->itn;
+        out->cg_rs = hs->rs;
+        out->max_dw = hs->max_dw;
+        out->updated = hs->ok;
+        out->graph_replayed = sp->graph && !c->graphs.empty();
+        out->ms = ms;
+    }
+    return 0;
 }
 
 int rbmgpu_sample_epoch(rbmgpu_ctx* c, uint32_t epoch) {
     if (!c) return -1;
-    rbmgpu_ctx* ctx = c;
     if (int rc = autotune(c)) return rc;
     if (int rc = ensure_samples(c, c->n_samples)) return rc;
-    HIPCK(hipMemsetAsync(c->d_obs, 0, c->n_samples * 2 * c->n_groups * sizeof(double), c->stream));
-    HIPCK(hipEventRecord(c->ev[0], c->stream));
-    int rc = 0;
-    if (c->Hn && c->lam > 0) {
-        HIPCK(launch_sample_mlp(learn_args(c, epoch), learn_k(c), c->stream));
-    } else {
-        rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, epoch, c->shape, c->block, c->n_samples, c->warmup)
-                                    : run_sample<double>(c, epoch, c->shape, c->block, c->n_samples, c->warmup);
-        if (!rc && eloc_after(c)) rc = run_eloc(c, c->n_samples);
-    }
-    if (rc) return rc;
-    HIPCK(hipEventRecord(c->ev[1], c->stream));
-    c->learned_epoch = c->Hn && c->lam > 0;
-    c->S = c->n_samples;
-    c->have_chains = true;
-    rc = c->prec == RBMGPU_FP32 ? run_build_O<float>(c) : run_build_O<double>(c);
-    if (rc) return rc;
-    HIPCK(hipEventRecord(c->ev[2], c->stream));
-    c->moments_valid = false;
-    return 0;
+    return sample_body(c, epoch);
 }
 
 int rbmgpu_load_samples(rbmgpu_ctx* c, const uint64_t* states, int64_t n) {
@@ -1586,7 +2126,7 @@ int rbmgpu_load_samples(rbmgpu_ctx* c, const uint64_t* states, int64_t n) {
     rbmgpu_ctx* ctx = c;
     if (int rc = ensure_samples(c, n)) return rc;
     HIPCK(hipMemcpyAsync(c->d_states, states, n * c->nw * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
-    HIPCK(hipMemsetAsync(c->d_obs, 0, n * 2 * c->n_groups * sizeof(double), c->stream));
+    if (int rc = zero_async(c, c->d_obs, n * 2 * c->n_groups * sizeof(double))) return rc;
     HIPCK(hipEventRecord(c->ev[0], c->stream));
     if (int rc = eval(c, c->d_states, n, nullptr, c->d_obs, c->d_T, nullptr, nullptr, 0)) return rc;
     HIPCK(hipEventRecord(c->ev[1], c->stream));
@@ -1658,6 +2198,7 @@ int rbmgpu_apply_S(rbmgpu_ctx* c, const double* x, double* y, double r1, double 
 
 int rbmgpu_learn_set_fraction(rbmgpu_ctx* c, double fraction) {
     if (!c || fraction < 0 || fraction > 1) return -1;
+    c->gen++;  // the sampler kernel and its arguments change
     if (!c->Hn && fraction > 0) return fail(c, "no proposal network (learn_hidden = 0)");
     c->lam = fraction;
     return 0;
@@ -1819,21 +2360,72 @@ int rbmgpu_time_sx(rbmgpu_ctx* c, int32_t reps, double* ms) {
 
 int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double r2, double rd,
                     int64_t max_iter, double rtol, int64_t* itn, double* rs) {
+    return rbmgpu_sr_solve_ex(c, RBMGPU_SOLVE_CG, b, x, r1, r2, rd, max_iter, rtol, itn, rs);
+}
+
+int rbmgpu_build_S(rbmgpu_ctx* c, double r1, double r2, double rd, double* S) {
+    if (!c || !S) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (int rc = build_S(c, r1, r2, rd, 0)) return rc;
+    HIPCK(hipMemcpyAsync(S, c->d_Smat, c->P * c->P * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rbmgpu_sr_solve_ex(rbmgpu_ctx* c, int32_t method, const double* b, double* x, double r1, double r2, double rd,
+                       int64_t max_iter, double rtol, int64_t* itn, double* rs) {
     if (!c || !b || !x) return -1;
+    if (method < RBMGPU_SOLVE_CG || method > RBMGPU_SOLVE_DIRECT) return fail(c, "unknown SR solve method");
     rbmgpu_ctx* ctx = c;
     if (!c->moments_valid)
         if (int rc = compute_moments(c)) return rc;
     if (int rc = ensure_cg(c)) return rc;
     if (int rc = ensure_sx(c)) return rc;
+    if (method == RBMGPU_SOLVE_DIRECT) {
+        if (int rc = build_S(c, r1, r2, rd, 1)) return rc;
+        std::vector<zd> Sm((size_t)c->P * c->P), bb(c->P), xx(c->P);
+        HIPCK(hipMemcpyAsync(Sm.data(), c->d_Smat, c->P * c->P * 16, hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < c->P; i++) bb[i] = zd(b[2 * i], b[2 * i + 1]);
+        if (int rc = dense_solve(c, Sm, bb, xx)) return rc;
+        double r2s = 0;  // |S x - b|^2 of the solution
+        for (int64_t i = 0; i < c->P; i++) {
+            zd acc = -bb[i];
+            for (int64_t j = 0; j < c->P; j++) acc += Sm[i + j * c->P] * xx[j];
+            r2s += std::norm(acc);
+        }
+        for (int64_t i = 0; i < c->P; i++) {
+            x[2 * i] = xx[i].real();
+            x[2 * i + 1] = xx[i].imag();
+        }
+        if (itn) *itn = 0;
+        if (rs) *rs = r2s;
+        return 0;
+    }
+    const bool dense = method == RBMGPU_SOLVE_CG_SINGLE;
+    if (dense)
+        if (int rc = build_S(c, r1, r2, rd, 0)) return rc;
     // cg_solver.cpp:48-60: max_iter = min(2P, max_iter), 0 -> 2P; rtol 0 -> 1e-16
     int64_t mi = std::min<int64_t>(2 * c->P, max_iter);
     if (mi == 0) mi = 2 * c->P;
     if (rtol == 0.) rtol = 1e-16;
+    const int nb_dense = (int)((c->P + 3) / 4);
+    // y = A x (the regularised S), CG step appended when cg
+    auto matvec = [&](const double* xin, double* yout, const CgStatus* st, bool cg) -> int {
+        if (!dense) return sx(c, xin, yout, r1, r2, rd, st, cg ? c->d_pdot : nullptr, cg);
+        HIPCK(launch_dense_mv(c->d_Smat, xin, yout, (int)c->P, cg ? c->d_pdot : nullptr, st, c->stream));
+        if (cg) {
+            hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, c->d_pdot, nb_dense, c->d_x, c->d_r,
+                               c->d_p, c->d_Ap, (int)c->P, c->d_st);
+            HIPCK(hipGetLastError());
+        }
+        return 0;
+    };
     HIPCK(hipMemcpyAsync(c->d_b, b, c->P * 16, hipMemcpyHostToDevice, c->stream));
     HIPCK(hipMemcpyAsync(c->d_x, x, c->P * 16, hipMemcpyHostToDevice, c->stream));
-    if (int rc = sx(c, c->d_x, c->d_Ap, r1, r2, rd, nullptr)) return rc;
+    if (int rc = matvec(c->d_x, c->d_Ap, nullptr, false)) return rc;
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1024), 0, c->stream, c->d_b, c->d_Ap, c->d_r, c->d_p, (int)c->P,
-                       c->d_st, mi, rtol);
+                       c->d_st, mi, rtol, (const DevPar*)nullptr);
     HIPCK(hipGetLastError());
     CgStatus st{};
     int64_t launched = 0;
@@ -1847,7 +2439,7 @@ int rbmgpu_sr_solve(rbmgpu_ctx* c, const double* b, double* x, double r1, double
     while (launched < mi) {
         nb = std::min<int64_t>(nb, mi - launched);
         for (int64_t i = 0; i < nb; i++)
-            if (int rc = sx(c, c->d_p, c->d_Ap, r1, r2, rd, c->d_st, c->d_pdot, true)) return rc;
+            if (int rc = matvec(c->d_p, c->d_Ap, c->d_st, true)) return rc;
         HIPCK(hipMemcpyAsync(&st, c->d_st, sizeof(CgStatus), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         launched += nb;

@@ -139,6 +139,22 @@ int rbmhost_pf_map(const rbmhost_model* m, int64_t* out, int64_t* rows) {
     return 0;
 }
 
+rbmhost_plugin* rbmhost_plugin_create(const char* kind, int64_t n, double p1, double p2, double p3) {
+    if (!kind || n < 0) return nullptr;
+    const std::string k(kind);
+    if (k == "adam") return reinterpret_cast<rbmhost_plugin*>(new optim::adam((size_t)n, p1, p2, p3));
+    if (k == "momentum") return reinterpret_cast<rbmhost_plugin*>(new optim::momentum((size_t)n, p1, p2));
+    return nullptr;
+}
+
+int rbmhost_plugin_apply(rbmhost_plugin* pl, double* dw, int64_t n, double lr) {
+    if (!pl || !dw || n < 0) return -1;
+    reinterpret_cast<optim::plugin*>(pl)->apply(reinterpret_cast<std::complex<double>*>(dw), (size_t)n, lr);
+    return 0;
+}
+
+void rbmhost_plugin_destroy(rbmhost_plugin* pl) { delete reinterpret_cast<optim::plugin*>(pl); }
+
 int rbmhost_step_ok(const double* dw, int64_t n, double* max_abs) {
     if (!dw || n < 0) return -1;
     return optim::step_ok((const std::complex<double>*)dw, (size_t)n, max_abs) ? 1 : 0;

@@ -117,19 +117,26 @@ int main(int argc, char** argv) {
         if (cfg.train) {
             epoch_log log(cfg.name, cfg.rbm_force);
             sampler::gpu_metropolis_sampler sampler(rbm, setup);
-            optimizer::stochastic_reconfiguration sr(rbm, sampler, cfg);
+            auto opt = optimizer::make_optimizer(rbm, sampler, cfg);
             std::cout << "Number of parameters: " << rbm.get_n_params() << std::endl;
             for (size_t i = rbm.get_n_updates(); i < cfg.n_epochs; i++) {
                 if (i > 0 && i % 100 == 0) rbm.save(cfg.name, true);
-                sampler.sample();
-                if (!cfg.noprogress) progress_bar(i + 1, cfg.n_epochs, sr.last_energy_per_site(), 'O');
-                if (!sr.optimize()) {
+                bool ok;
+                if (opt->fused()) {
+                    // sampling and the SR step as one device epoch (one graph replay)
+                    ok = opt->train_step();
+                } else {
+                    sampler.sample();
+                    if (!cfg.noprogress) progress_bar(i + 1, cfg.n_epochs, opt->last_energy_per_site(), 'O');
+                    ok = opt->optimize();
+                }
+                if (!ok) {
                     std::cerr << "\nFATAL: Massive step detected." << std::endl;
                     rc = 55;
                     break;
                 }
-                log.line(sr.log_values);
-                if (!cfg.noprogress) progress_bar(i + 1, cfg.n_epochs, sr.last_energy_per_site(), 'S');
+                log.line(opt->log_values);
+                if (!cfg.noprogress) progress_bar(i + 1, cfg.n_epochs, opt->last_energy_per_site(), 'S');
             }
             if (!cfg.noprogress) std::cout << std::endl;
             rbm.save(cfg.name);

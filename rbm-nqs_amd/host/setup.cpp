@@ -170,7 +170,10 @@ rbmgpu_config setup::gpu_config() const {
     c.seed = cfg_.seed;
     c.precision = cfg_.gpu_precision;
     c.lanes_per_chain = cfg_.gpu_lanes;
-    c.keep_o = cfg_.gpu_keep_o ? 1 : 0;
+    // the explicit-S solvers (cg-single, direct and possibly cg-dynamic) read O
+    const auto& m = cfg_.opt_sr_method;
+    const bool explicit_S = cfg_.opt_type == "SR" && (m == "cg-single" || m == "direct" || m == "cg-dynamic");
+    c.keep_o = (cfg_.gpu_keep_o || explicit_S) ? 1 : 0;
     c.learn_hidden = cfg_.learn_hidden;
     c.learn_fraction = cfg_.learn_fraction;
     c.cluster_kmax = cluster_kmax_;

@@ -22,6 +22,18 @@ U64P = C.POINTER(C.c_uint64)
 FP32, FP64 = 32, 64
 
 
+class SrParams(C.Structure):
+    _fields_ = [("lr", C.c_double), ("reg1", C.c_double), ("reg2", C.c_double), ("reg1_delta", C.c_double),
+                ("max_iter", C.c_int64), ("rtol", C.c_double), ("graph", C.c_int32)]
+
+
+class EpochStats(C.Structure):
+    _fields_ = [("e_mean", C.c_double * 2), ("e_var", C.c_double), ("e_var_binned", C.c_double),
+                ("e_tau", C.c_double), ("e_std", C.c_double), ("n_bins", C.c_int64), ("acceptance", C.c_double),
+                ("cg_itn", C.c_int64), ("cg_rs", C.c_double), ("max_dw", C.c_double), ("updated", C.c_int32),
+                ("graph_replayed", C.c_int32), ("ms", C.c_double)]
+
+
 class Moments(C.Structure):
     _fields_ = [("e_mean", C.c_double * 2), ("e_var", C.c_double), ("e_var_binned", C.c_double),
                 ("e_tau", C.c_double), ("e_std", C.c_double), ("n_bins", C.c_int64),
@@ -61,11 +73,13 @@ def exported_symbols():
                 "rbmgpu_update_weights", "rbmgpu_sample_epoch", "rbmgpu_load_samples",
                 "rbmgpu_reduce", "rbmgpu_sr_solve", "rbmgpu_apply_S", "rbmgpu_eval_logpsi",
                 "rbmgpu_psi_ratio", "rbmgpu_get_samples", "rbmgpu_n_samples",
-                "rbmgpu_kernel_times", "rbmgpu_synchronize"],
+                "rbmgpu_kernel_times", "rbmgpu_synchronize", "rbmgpu_train_epoch",
+                "rbmgpu_sr_solve_ex", "rbmgpu_build_S"],
         "host": ["rbmhost_create", "rbmhost_destroy", "rbmhost_info", "rbmhost_bonds",
                  "rbmhost_perms", "rbmhost_terms", "rbmhost_exact_energy", "rbmhost_use_evaluation",
                  "rbmhost_set_n_samples", "rbmhost_initial_weights", "rbmhost_gpu_create",
-                 "rbmhost_clusters", "rbmhost_step_ok", "rbmhost_pf_map"],
+                 "rbmhost_clusters", "rbmhost_step_ok", "rbmhost_pf_map", "rbmhost_plugin_create",
+                 "rbmhost_plugin_apply", "rbmhost_plugin_destroy"],
     }
 
 
@@ -77,6 +91,36 @@ def step_ok(dw):
     if rc < 0:
         raise RuntimeError("rbmhost_step_ok: bad arguments")
     return rc == 1, mx.value
+
+
+class Plugin:
+    """optimizer.plugin of the host layer (plugin.cpp:30-77): "adam" (beta1,
+    beta2, eps) or "momentum" (alpha, dialup)"""
+
+    def __init__(self, kind, n, p1=0.0, p2=0.0, p3=0.0):
+        h = libs()[1]
+        h.rbmhost_plugin_create.restype = C.c_void_p
+        h.rbmhost_plugin_create.argtypes = [C.c_char_p, C.c_int64, C.c_double, C.c_double, C.c_double]
+        h.rbmhost_plugin_destroy.argtypes = [C.c_void_p]
+        h.rbmhost_plugin_destroy.restype = None
+        self._h = h.rbmhost_plugin_create(kind.encode(), n, p1, p2, p3)
+        if not self._h:
+            raise RuntimeError(f"rbmhost: unknown plugin '{kind}'")
+        self.n = n
+
+    def apply(self, dw, lr):
+        d = np.array(dw, dtype=np.complex128)
+        assert d.size == self.n
+        rc = libs()[1].rbmhost_plugin_apply(C.c_void_p(self._h), d.ctypes.data_as(DP), C.c_int64(d.size),
+                                            C.c_double(lr))
+        if rc != 0:
+            raise RuntimeError("rbmhost_plugin_apply failed")
+        return d
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            libs()[1].rbmhost_plugin_destroy(C.c_void_p(self._h))
+            self._h = None
 
 
 def _p(a, t):
@@ -271,6 +315,33 @@ class Gpu:
         self._chk(libs()[0].rbmgpu_apply_S(self.c, _p(x, DP), _p(y, DP), C.c_double(r1), C.c_double(r2),
                                            C.c_double(rd)))
         return y
+
+    def train_epoch(self, epoch, lr, reg1, reg2, reg1_delta, max_iter=0, rtol=0.0, graph=True):
+        """one SR epoch on the device (rbmgpu_train_epoch); returns its statistics"""
+        p = SrParams(lr, reg1, reg2, reg1_delta, max_iter, rtol, int(graph))
+        st = EpochStats()
+        self._chk(libs()[0].rbmgpu_train_epoch(self.c, C.c_uint32(epoch), C.byref(p), C.byref(st)))
+        return dict(mean=complex(st.e_mean[0], st.e_mean[1]), var=st.e_var, var_binned=st.e_var_binned,
+                    tau=st.e_tau, std=st.e_std, n_bins=st.n_bins, acc=st.acceptance, itn=st.cg_itn, rs=st.cg_rs,
+                    max_dw=st.max_dw, updated=bool(st.updated), graph=bool(st.graph_replayed), ms=st.ms)
+
+    def sr_solve_ex(self, method, b, x0, r1, r2, rd, max_iter=0, rtol=0.0):
+        """rbmgpu_sr_solve_ex: method 0 matrix-free CG, 1 CG on the explicit S
+        (cg-single), 2 direct"""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.array(x0, dtype=np.complex128)
+        itn = C.c_int64()
+        rs = C.c_double()
+        self._chk(libs()[0].rbmgpu_sr_solve_ex(self.c, C.c_int32(method), _p(b, DP), _p(x, DP), C.c_double(r1),
+                                               C.c_double(r2), C.c_double(rd), C.c_int64(max_iter),
+                                               C.c_double(rtol), C.byref(itn), C.byref(rs)))
+        return x, itn.value, rs.value
+
+    def build_S(self, r1, r2, rd):
+        """explicit S with the CG regularisation (P x P)"""
+        S = np.zeros(self.P * self.P, dtype=np.complex128)
+        self._chk(libs()[0].rbmgpu_build_S(self.c, C.c_double(r1), C.c_double(r2), C.c_double(rd), _p(S, DP)))
+        return S.reshape(self.P, self.P, order="F")
 
     def sr_solve(self, b, x0, r1, r2, rd, max_iter=0, rtol=0.0):
         b = np.ascontiguousarray(b, dtype=np.complex128)
