@@ -181,6 +181,19 @@ int rbmgpu_build_S(rbmgpu_ctx* ctx, double r1, double r2, double rd, double* S);
 int rbmgpu_apply_S(rbmgpu_ctx* ctx, const double* x, double* y, double r1,
                    double r2, double rd);
 
+/* Allocation-free variant of rbmgpu_apply_S.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  x and y are DEVICE
+ * pointers (P complex doubles each); the call is asynchronous on the
+ * context's stream (rbmgpu_synchronize waits for it).  The S.x block
+ * partials go to a static per-device scratch of 8 MiB (at P = 2144 in FP32
+ * with the structured S.x about 4.5 MB); a sample set whose partials need
+ * more is rejected with an error.  The moments of the current sample set
+ * must exist (rbmgpu_reduce first).  Every call on one GPU shares that
+ * scratch, so calls that overlap on one GPU (several contexts or streams)
+ * race. */
+int rbmgpu_apply_S_noalloc(rbmgpu_ctx* ctx, const double* x, double* y, double r1, double r2, double rd);
+
 /* parity hooks on host-given states */
 int rbmgpu_eval_logpsi(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n,
                        double* logpsi);

@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <complex>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -130,9 +131,20 @@ struct rbmgpu_ctx {
     float* d_tw = nullptr;
     bool learned_epoch = false;  // the last epoch ran the learned sampler
     int64_t mlp_n() const { return (int64_t)Hn * N + Hn + (int64_t)N * Hn + N; }
+    // device address of the static scratch array on this context's device
+    void* scratch_dev = nullptr;
     std::string err;
 };
 
+// Static device scratch of the allocation-free S.x (rbmgpu_apply_S_noalloc):
+// the S.x block partials.  Each GPU the library runs on holds one copy
+// until the process ends; every call on that GPU shares it.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// device (g_scratch), never through the array's name.
+constexpr int64_t kScratchBytes = (int64_t)8 << 20;  // 8 MiB
+static __device__ __align__(256) char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch[64] = {};
 
 namespace {
 
@@ -1849,6 +1861,20 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     HIPCK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     c->lds_limit = (size_t)lds;
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        // the static scratch's address on this device, looked up once per
+        // device with the device current
+        if (dev < 0 || dev >= 64) return bad("device ordinal out of range for the scratch table");
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[dev]) {
+            void* sp = nullptr;
+            hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !sp)
+                return bad(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            g_scratch[dev] = sp;
+        }
+        c->scratch_dev = g_scratch[dev];
+    }
     // timing-only events: no system-scope fence (no cache writeback and
     // invalidation between the kernels they bracket); results are read
     // after a stream synchronisation
@@ -2194,6 +2220,20 @@ int rbmgpu_apply_S(rbmgpu_ctx* c, const double* x, double* y, double r1, double 
     HIPCK(hipStreamSynchronize(c->stream));
     c->times[3] = elapsed(c, 6, 7);
     return 0;
+}
+
+int rbmgpu_apply_S_noalloc(rbmgpu_ctx* c, const double* d_x, double* d_y, double r1, double r2, double rd) {
+    if (!c || !d_x || !d_y) return -1;
+    if (!c->scratch_dev) return fail(c, "rbmgpu_apply_S_noalloc: no scratch address for this device");
+    if (!c->moments_valid)
+        return fail(c, "rbmgpu_apply_S_noalloc: no moments of the current sample set (call rbmgpu_reduce first)");
+    int64_t blocks, bytes;
+    if (int rc = sx_layout(c, &blocks, &bytes)) return rc;
+    if (bytes > kScratchBytes)
+        return fail(c, "rbmgpu_apply_S_noalloc: the S.x block partials need " + std::to_string(bytes) +
+                           " bytes, more than the static scratch's " + std::to_string(kScratchBytes));
+    c->sx_blocks = blocks;
+    return sx(c, d_x, d_y, r1, r2, rd, nullptr, nullptr, false, c->scratch_dev);
 }
 
 int rbmgpu_learn_set_fraction(rbmgpu_ctx* c, double fraction) {

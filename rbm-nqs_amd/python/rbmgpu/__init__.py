@@ -73,7 +73,7 @@ def exported_symbols():
                 "rbmgpu_update_weights", "rbmgpu_sample_epoch", "rbmgpu_load_samples",
                 "rbmgpu_reduce", "rbmgpu_sr_solve", "rbmgpu_apply_S", "rbmgpu_eval_logpsi",
                 "rbmgpu_psi_ratio", "rbmgpu_get_samples", "rbmgpu_n_samples",
-                "rbmgpu_kernel_times", "rbmgpu_synchronize", "rbmgpu_train_epoch",
+                "rbmgpu_kernel_times", "rbmgpu_synchronize", "rbmgpu_apply_S_noalloc", "rbmgpu_train_epoch",
                 "rbmgpu_sr_solve_ex", "rbmgpu_build_S"],
         "host": ["rbmhost_create", "rbmhost_destroy", "rbmhost_info", "rbmhost_bonds",
                  "rbmhost_perms", "rbmhost_terms", "rbmhost_exact_energy", "rbmhost_use_evaluation",
@@ -315,6 +315,12 @@ class Gpu:
         self._chk(libs()[0].rbmgpu_apply_S(self.c, _p(x, DP), _p(y, DP), C.c_double(r1), C.c_double(r2),
                                            C.c_double(rd)))
         return y
+
+    def apply_S_noalloc(self, x_dev, y_dev, r1, r2, rd):
+        """rbmgpu_apply_S_noalloc on DEVICE addresses x_dev, y_dev (ints, P
+        complex doubles each); asynchronous on the context's stream."""
+        self._chk(libs()[0].rbmgpu_apply_S_noalloc(self.c, C.c_void_p(x_dev), C.c_void_p(y_dev), C.c_double(r1),
+                                                   C.c_double(r2), C.c_double(rd)))
 
     def train_epoch(self, epoch, lr, reg1, reg2, reg1_delta, max_iter=0, rtol=0.0, graph=True):
         """one SR epoch on the device (rbmgpu_train_epoch); returns its statistics"""
