@@ -1,20 +1,30 @@
-"""Benchmark: SR training epochs of the toric code 4x4 (N = 32 spins),
-basic complex RBM alpha = 2 (P = 32 + 64 + 2048 = 2144), 4096 Metropolis
-chains x 64 kept samples (262 144 samples per epoch), 100 warm-up sweeps,
-bond_flips 0.5, matrix-free SR solved by CG with FP64 vectors
-(sr.max_iterations = 100 as in examples/toric/toric.ini).
+"""Benchmark of the VMC epoch on one MI355X.
 
-One step = one full epoch: sample (persistent chain kernel with fused local
-energies) -> O assembly -> reduction -> F -> CG solve -> weight update, all
-on one GPU.  Inputs (weights, tables) are resident in HBM before timing.
+Headline workload (`value`): Kitaev honeycomb 6x6 (N = 72 spins), symmetric
+RBM alpha = 4 with the 72 translation x rotation symmetries (P = 293),
+16384 Metropolis chains x 8 kept samples (131 072 samples per epoch), 100
+warm-up sweeps, bond_flips 0.5, FP32 sampler and O contractions, SR solved
+by CG with FP64 vectors (sr.max_iterations = 100).  One step = one whole SR
+epoch on the device (rbmgpu_train_epoch: sample -> local energies ->
+moments -> F -> CG -> guard -> update), captured once in a HIP graph and
+replayed.  Weights: the checkpoint rbm-nqs_amd/examples/kitaev6_sym.rbm
+(trained by the `rbm` driver on kitaev6_sym.ini) when present, else
+N(0, 0.05^2) from the reference's initialiser.
 
-Prints one JSON line (see the driver contract) with `roofline` for the
-dominant kernel (the S.x product of the CG) and `cpu_baseline` (the FP64
-CPU oracle on a bounded sample of the same workload, rank 0 only).
+`value` = kept samples per second of whole epochs; `effective_samples_per_s`
+= value / (1 + 2 tau) with the reference's binning tau
+(aggregator.cpp:120-128).  Side lines in the same JSON object: the toric
+4x4 workload in FP32 and FP64, the Pfaffian workload (Kitaev 4x4 + Pfaffian,
+8192 chains), the same headline at N(0, 1e-4^2) weights, plain vs learned
+proposal, `roofline` for the dominant kernel, `cpu_baseline` (the FP64 CPU
+oracle on a bounded sample of the headline workload) and its calibration
+against the reference's own per-rank numbers (toric n = 3, 4, 6 plogs).
 """
 import argparse
 import json
 import os
+import platform
+import struct
 import sys
 import time
 
@@ -25,12 +35,37 @@ import numpy as np  # noqa: E402
 
 METRIC = "samples/s and SR epoch ms at fixed energy error, plain vs learned proposal"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector and matrix)
+CHECKPOINT = os.path.join(ROOT, "rbm-nqs_amd", "examples", "kitaev6_sym.rbm")
+E_EXACT_KITAEV6 = -0.78823570803603  # kitaev.cpp:66-100 at 6x6
 
-WORKLOAD = dict(model="toric", n_cells=4, rbm="basic", alpha=2, chains=4096, samples_per_chain=64,
-                warmup=100, bond_flips=0.5, sr_max_iterations=100, precision=32, weights_std=0.05)
+
+def kitaev6_spec(chains=16384, spc=8, warmup=100, weights=0.05, learn=0, max_iter=100):
+    s = f"""
+[model]
+type = kitaev
+n_cells = 6
+J = -1
+symmetry = 0.5
+[rbm]
+type = symmetry
+alpha = 4
+weights = {weights}
+[sampler]
+n_samples = {chains * spc}
+metropolis.n_chains = {chains}
+metropolis.n_warmup_steps = {warmup}
+metropolis.bond_flips = 0.5
+[optimizer]
+learning_rate = 0.01
+sr.max_iterations = {max_iter}
+"""
+    if learn:
+        s += f"[learn]\nhidden = {learn}\nfraction = 0.5\n"
+    return s
 
 
-def spec(chains, samples_per_chain, warmup=100, max_iter=100):
+def toric4_spec(chains=4096, spc=64, warmup=100):
     return f"""
 [model]
 type = toric
@@ -41,14 +76,36 @@ type = basic
 alpha = 2
 weights = 0.05
 [sampler]
-n_samples = {chains * samples_per_chain}
+n_samples = {chains * spc}
 metropolis.n_chains = {chains}
 metropolis.n_warmup_steps = {warmup}
-metropolis.n_steps_per_sample = 1
 metropolis.bond_flips = 0.5
 [optimizer]
-learning_rate = 0.01
-sr.max_iterations = {max_iter}
+sr.max_iterations = 100
+"""
+
+
+def pfaffian_spec(chains=8192, spc=8, warmup=100):
+    return f"""
+[model]
+type = kitaev
+n_cells = 4
+J = -1
+symmetry = 0.5
+[rbm]
+type = basic
+alpha = 2
+weights = 0.05
+pfaffian = true
+pfaffian.weights = 0.01
+[sampler]
+n_samples = {chains * spc}
+metropolis.n_chains = {chains}
+metropolis.n_warmup_steps = {warmup}
+metropolis.bond_flips = 0.5
+pfaffian_refresh = 64
+[optimizer]
+sr.max_iterations = 100
 """
 
 
@@ -66,141 +123,135 @@ class Decay:
         return self.v
 
 
-def gpu_epochs(steps, warmup):
+def read_rbm(path, m):
+    """a .rbm checkpoint (rbm_base.cpp:219-288, eigen_fstream.cpp:25-51) as
+    the parameter vector [v | h | W column-major]"""
+    data = open(path, "rb").read()
+    off, mats = 0, []
+    for _ in range(3):
+        r, c = struct.unpack_from("<QQ", data, off)
+        off += 16
+        mats.append(np.frombuffer(data, dtype=np.complex128, count=r * c, offset=off).copy())
+        off += 16 * r * c
+    W, h, v = mats
+    p = np.concatenate([v, h, W])
+    assert p.size == m.P, (p.size, m.P)
+    return p
+
+
+def epochs(spec, precision, steps, warmup, params=None, graph=True, seed=421390484):
+    """whole SR epochs on the device; returns timing and statistics"""
     import rbmgpu
 
-    m = rbmgpu.Model(spec(WORKLOAD["chains"], WORKLOAD["samples_per_chain"]))
-    g = m.gpu(precision=WORKLOAD["precision"])
-    g.set_params(m.initial_weights(421390484))
+    m = rbmgpu.Model(spec)
+    g = m.gpu(precision=precision)
+    g.set_params(m.initial_weights(seed) if params is None else params)
     lr, k1, k2, kd = Decay(1e-2, 1e-2, 1.0), Decay(1, 1e-4, 0.9), Decay(1e-3, 1e-6, 0.9), Decay(1e-2, 1e-2, 1.0)
-    dw = np.zeros(g.P, dtype=np.complex128)
+    mi = 100
     log = []
 
-    def epoch(e):
-        nonlocal dw
-        g.sample_epoch(epoch=e)
-        r = g.reduce()
-        F = r["eo"] - r["mean"] * r["o_mean"].conj()
-        rate = lr.get()
-        r1, r2, rd = k1.get(), k2.get(), kd.get()
-        dw, itn, rs = g.sr_solve(F, dw, r1, r2, rd, max_iter=WORKLOAD["sr_max_iterations"])
-        dw *= rate
-        g.update_weights(dw)
-        log.append(dict(E=r["mean"].real / m.N, std=r["std"] / m.N, acc=r["acc"], tau=r["tau"], itn=itn))
+    def one(e):
+        st = g.train_epoch(e, lr.get(), k1.get(), k2.get(), kd.get(), max_iter=mi, graph=graph)
+        log.append(st)
 
     for e in range(warmup):
-        epoch(e)
+        one(e)
     g.synchronize()
     t0 = time.perf_counter()
     for e in range(warmup, warmup + steps):
-        epoch(e)
+        one(e)
     g.synchronize()
     dt = time.perf_counter() - t0
-    times = g.kernel_times()
-    return m, g, dt, log, times
+    timed = log[warmup:]
+    tau = float(np.mean([s["tau"] for s in timed]))
+    S = m.n_samples
+    out = {"ms_per_epoch": 1e3 * dt / steps, "samples_per_s": S * steps / dt, "samples_per_epoch": S,
+           "tau_int": tau, "effective_samples_per_s": S * steps / dt / (1 + 2 * max(tau, 0.0)),
+           "energy_per_site": timed[-1]["mean"].real / m.N, "energy_stddev_per_site": timed[-1]["std"] / m.N,
+           "acceptance": float(np.mean([s["acc"] for s in timed])), "cg_iterations": timed[-1]["itn"],
+           "graph_replayed": all(s["graph"] for s in timed) if graph else False,
+           "launch_shape": list(g.launch_shape()), "structured_sx": g.structured()}
+    return m, g, out
 
 
-def cpu_baseline():
-    """FP64 CPU oracle epoch (same model, bounded: 64 chains x 32 samples)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-
-    chains, spc = 512, 32
-    o = po.Oracle(spec(chains, spc))
-    rng = np.random.default_rng(421390484)
-    o.set_params(WORKLOAD["weights_std"] * (rng.normal(size=o.P) + 1j * rng.normal(size=o.P)))
-    threads = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    st = o.epoch(seed=421390484, epoch=0, threads=threads, use_lut=True)
-    dt = time.perf_counter() - t0
-    S = chains * spc
-    return dict(value=S / dt, unit="samples/s", cores=threads, kind="port",
-                sample=f"1 SR epoch of the same model with {chains} chains x {spc} samples "
-                       f"({S} samples, 100 warm-up sweeps, CG 100 iterations); FP64 oracle with the "
-                       f"reference's lncosh cache, chains on {threads} OpenMP threads, CG on 1 thread; "
-                       f"sampling {st['t_sample']:.2f} s, SR {st['t_opt']:.2f} s")
+def sampler_flops(m, n_chains, sweeps, kept, bond_flips=0.5, n_offdiag_terms=None, flips_per_term=2):
+    """algorithmic FP32 flops of one sampling epoch (the work model of the
+    sampler): a slot proposes a bond (2 flips, probability bond_flips,
+    consuming two slots) or a single flip, so a sweep of N slots makes
+    N (1 - b) / (1 + b) single and N b / (1 + b) bond proposals; per unit
+    24 flops for a single-flip ratio and 40 for a two-flip one (composition
+    of the cosh / sinh factors, the ratio, the group product and the
+    accepted-move update); each kept sample evaluates its off-diagonal
+    local terms at 8 m flops per unit (m flipped sites)"""
+    N, H = m.N, m.H
+    b = bond_flips
+    singles = N * (1 - b) / (1 + b)
+    bonds = N * b / (1 + b)
+    per_sweep = (24 * singles + 40 * bonds) * H
+    nt = n_offdiag_terms if n_offdiag_terms is not None else N
+    per_sample = nt * 8 * flips_per_term * H
+    return n_chains * sweeps * per_sweep + n_chains * kept * per_sample
 
 
-# HBM bytes per k_struct launch of this workload from two rocprofv3 --pmc
-# runs (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the CG
-# matvecs), each scaled by the factor a plain 1 GiB device copy gives
-# (bench.py --copy-calibration 1024: FETCH_SIZE reads 524302 KiB for
-# 1048576 KiB read -> x2.0; WRITE_SIZE reads 1048576 KiB -> x1.0):
-# 61817 KiB x 2.0 + 3968 KiB x 1.0
+def sampler_roofline(g, m, chains, spc, warmup):
+    """roofline of the dominant kernel of the headline epoch: the sampler
+    (k_sample: all chains, warm-up and kept sweeps, in-sampler local
+    energies).  It runs on the FP32 VALU: achieved = the algorithmic flops
+    of sampler_flops / the kernel's duration between HIP events on the
+    context's stream (an eager sampling epoch after the timed loop)."""
+    ms = []
+    for e in range(3):
+        g.sample_epoch(epoch=900 + e)
+        g.synchronize()
+        ms.append(g.kernel_times()["sample"])
+    t = float(np.median(ms))
+    flops = sampler_flops(m, chains, warmup + spc, spc)
+    achieved = flops / (t * 1e-3) / 1e12
+    return {"kernel": "k_sample (Metropolis chains of the epoch with their local energies)", "bound": "mfma",
+            "issue": "FP32 VALU (157.3 TFLOP/s is MI355X's FP32 peak, vector and matrix alike)",
+            "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None, "flops_per_launch": flops, "launch_ms": t}
+
+
+# HBM bytes per k_struct launch of the toric 4x4 workload from two rocprofv3
+# --pmc runs (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by
+# the factor a plain 1 GiB device copy gives (FETCH x2.0, WRITE x1.0)
 STRUCT_TRAFFIC_BYTES = int((61816.8 * 1048576 / 524302.4 + 3968.4) * 1024)
 
 
-def sx_roofline(g, m, S, sx_ms):
-    """Roofline of one S.x of the CG.
-
-    Structured (basic RBM, FP32): k_struct reads T (S x M complex FP32), one
-    state word per sample and x once per launch -- the algorithmic bytes,
-    S (8 M + 8) + 16 P.  Its two GEMMs (G = Sigma X, Z = Sigma^T diag(y)
-    conj(T): 8 S N M flops, plus 26 S M + 8 S N for y and the column sums)
-    run on bf16 MFMA with exact three-term splits (three MFMAs per product
-    at 16x the FP32 MFMA rate), so HBM is the binding roofline;
-    `fp32_equiv_tflops` is the FP32 flop rate of the same launch.
-    Materialised O: one pass over O (P x S complex FP32), HBM bound."""
-    M = (m.P - m.N) // (m.N + 1)  # P = N + M + N M for the basic RBM
-    if g.structured():
-        nbytes = S * (8 * M + 8) + 16 * m.P
-        flops = 8 * S * m.N * M + 26 * S * M + 8 * S * m.N
-        achieved = nbytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
-        return {"kernel": "k_struct (one S.x of the CG: G = Sigma X, Z = Sigma^T diag(y) conj(T))",
-                "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": STRUCT_TRAFFIC_BYTES, "algorithmic_bytes": nbytes,
-                "fp32_equiv_tflops": flops / (sx_ms * 1e-3) / 1e12 if sx_ms > 0 else 0.0,
-                "flops_per_launch": flops, "launch_ms": sx_ms}
-    o_bytes = g.P * S * 8
-    achieved = o_bytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
-    return {"kernel": "k_sx_partial (one S.x of the CG over the materialised O)", "bound": "hbm",
-            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "launch_ms": sx_ms}
+def struct_roofline(g, m, S):
+    """the structured S.x of the toric 4x4 line (k_struct), against HBM:
+    algorithmic bytes S (8 M + 8) + 16 P per launch"""
+    sx_ms = g.time_sx(20)
+    M = (m.P - m.N) // (m.N + 1)
+    nbytes = S * (8 * M + 8) + 16 * m.P
+    achieved = nbytes / (sx_ms * 1e-3) / 1e9 if sx_ms > 0 else 0.0
+    return {"kernel": "k_struct", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": STRUCT_TRAFFIC_BYTES, "algorithmic_bytes": nbytes,
+            "launch_ms": sx_ms}
 
 
-def c3_spec(learn, chains=16384, spc=8, warmup=100, hidden=256, fraction=0.5):
-    s = f"""
-[model]
-type = kitaev
-n_cells = 6
-J = -1
-symmetry = 0.5
-[rbm]
-type = symmetry
-alpha = 4
-weights = 0.05
-[sampler]
-n_samples = {chains * spc}
-metropolis.n_chains = {chains}
-metropolis.n_warmup_steps = {warmup}
-"""
-    if learn:
-        s += f"[learn]\nhidden = {hidden}\nfraction = {fraction}\n"
-    return s
-
-
-def plain_vs_learned(epochs=3, train_epochs=5, target=1e-4):
-    """Kitaev 6x6 symmetric RBM alpha=4: plain Metropolis (single and bond
-    flips) against the learned cluster proposal (72 -> 256 -> 72 network,
-    bf16 MFMA, trained for train_epochs REINFORCE/Adam steps on its own
-    records first).  Per epoch: sampler time, binned tau and std of E/N;
-    time to a fixed error of E/N = epoch time x (std / target)^2."""
+def plain_vs_learned(params, epochs_=3, train_epochs=5, target=1e-4):
+    """headline model at the headline weights: plain Metropolis (single and
+    bond flips) against the learned cluster proposal (72 -> 256 -> 72,
+    bf16 MFMA, trained train_epochs REINFORCE/Adam steps on its own
+    records first).  Time to a fixed error of E/N = epoch time x
+    (std / target)^2."""
     import rbmgpu
 
-    out = {"workload": "Kitaev honeycomb 6x6 (N=72), symmetric RBM alpha=4 (P=293), weights N(0,0.05^2), "
-                       "16384 chains x 8 samples, 100 warm-up sweeps, FP32 sampler",
-           "target_error_per_site": target}
-    for learn in (False, True):
-        m = rbmgpu.Model(c3_spec(learn))
+    out = {"workload": "Kitaev 6x6 symmetric RBM alpha=4, 16384 chains x 8 samples, 100 warm-up sweeps, "
+                       "headline weights, sampling epochs only", "target_error_per_site": target}
+    for learn in (0, 256):
+        m = rbmgpu.Model(kitaev6_spec(learn=learn))
         g = m.gpu(precision=32)
-        g.set_params(m.initial_weights(421390484))
+        g.set_params(params if params is not None else m.initial_weights(421390484))
         tr = []
         if learn:
             for e in range(train_epochs):
                 g.sample_epoch(epoch=1000 + e)
                 tr.append(g.learn_train(lr=3e-3, beta=0.01, batch=16384))
         ms, taus, stds, accs, es = [], [], [], [], []
-        for e in range(epochs):
+        for e in range(epochs_):
             g.sample_epoch(epoch=e)
             r = g.reduce(want_o=False)
             ms.append(g.kernel_times()["sample"])
@@ -215,9 +266,96 @@ def plain_vs_learned(epochs=3, train_epochs=5, target=1e-4):
                "energy_per_site": float(np.mean(es)), "energy_stddev_per_site": sd,
                "ms_to_target_error": t * (sd / target) ** 2}
         if learn:
-            rec.update({"hidden": 256, "learned_fraction": 0.5, "training_epochs": train_epochs,
+            rec.update({"hidden": learn, "learned_fraction": 0.5, "training_epochs": train_epochs,
                         "learned_acceptance": tr[-1]["accepted"] / max(tr[-1]["proposed"], 1)})
         out["learned" if learn else "plain"] = rec
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() else n
+
+
+def cpu_baseline(params):
+    """the FP64 CPU oracle (a port of the reference's algorithm: sequential
+    chains per worker, per-proposal theta update + lncosh with the
+    reference's state-keyed cache, bond flips, matrix-free CG) on the
+    headline model and weights, bounded: 512 chains x 8 samples (the GPU's
+    samples per chain and warm-up), chains and CG S.x on every host thread"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    import rbmgpu
+
+    chains, spc = 512, 8
+    spec = kitaev6_spec(chains, spc)
+    o = po.Oracle(spec)
+    o.set_params(params if params is not None else rbmgpu.Model(spec).initial_weights(421390484))
+    threads = host_threads()
+    t0 = time.perf_counter()
+    st = o.epoch(seed=421390484, epoch=0, threads=threads, use_lut=True)
+    dt = time.perf_counter() - t0
+    S = chains * spc
+    return dict(value=S / dt, unit="samples/s", cores=threads, kind="port", cpu_model=cpu_model(),
+                effective_value=S / dt / (1 + 2 * max(st["tau"], 0.0)), tau_int=st["tau"],
+                sample=f"1 SR epoch of the headline model and weights with {chains} chains x {spc} samples "
+                       f"({S} samples, 100 warm-up sweeps, CG {int(st['itn'])} iterations); FP64 oracle with the "
+                       f"reference's lncosh cache on {threads} threads (chains and CG); sampling "
+                       f"{st['t_sample']:.2f} s, SR {st['t_opt']:.2f} s")
+
+
+# the reference's own per-rank figures (examples/toric/backup/*.plog: symmetric
+# RBM alpha = 1, model.symmetry = 1, one chain per rank, 5 sweeps per kept
+# sample, about 200 warm-up sweeps per epoch): kept samples per second per
+# rank of the Sampling region
+PLOG = {3: (800, 11.23e3), 4: (800, 4.29e3), 6: (400, 1.24e3)}
+
+
+def calibration():
+    """the oracle on one thread with the plog settings, as a ratio to the
+    reference's published per-rank sampling rate"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    out = {}
+    for n, (samples, ref) in PLOG.items():
+        spec = f"""
+[model]
+type = toric
+n_cells = {n}
+J = -1
+symmetry = 1
+[rbm]
+type = symmetry
+alpha = 1
+weights = 0.01
+[sampler]
+n_samples = {samples}
+metropolis.n_chains = 1
+metropolis.n_steps_per_sample = 5
+metropolis.n_warmup_steps = 200
+metropolis.bond_flips = 0.2
+[optimizer]
+sr.max_iterations = 100
+"""
+        o = po.Oracle(spec)
+        rng = np.random.default_rng(1)
+        o.set_params(0.01 * (rng.normal(size=o.P) + 1j * rng.normal(size=o.P)))
+        st = o.epoch(seed=1, epoch=0, threads=1, use_lut=True)
+        rate = samples / st["t_sample"]
+        out[f"toric_n{n}"] = {"oracle_samples_per_s_1core": rate, "reference_samples_per_s_per_rank": ref,
+                              "ratio": rate / ref, "oracle_sr_ms": 1e3 * st["t_opt"]}
     return out
 
 
@@ -235,14 +373,13 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
 
 
-def sampler_only(spc):
-    """Sampling epochs alone on the default workload's chains and model with
-    spc kept samples per chain (for counter runs under rocprofv3 --pmc: the
-    difference of two spc values isolates the kept-sample local energies)."""
+def sampler_only(spc, workload):
+    """sampling epochs alone (for counter runs under rocprofv3 --pmc)"""
     import rbmgpu
 
-    m = rbmgpu.Model(spec(WORKLOAD["chains"], spc))
-    g = m.gpu(precision=WORKLOAD["precision"])
+    spec = kitaev6_spec(spc=spc) if workload == "kitaev6" else toric4_spec(spc=spc)
+    m = rbmgpu.Model(spec)
+    g = m.gpu(precision=32)
     g.set_params(m.initial_weights(421390484))
     ms = []
     for e in range(3):
@@ -250,8 +387,15 @@ def sampler_only(spc):
         g.sample_epoch(epoch=e)
         g.synchronize()
         ms.append(1e3 * (time.perf_counter() - t0))
-    print(json.dumps({"sampler_only": {"samples_per_chain": spc, "chains": WORKLOAD["chains"],
-                                       "epoch_ms": ms}}))
+    print(json.dumps({"sampler_only": {"workload": workload, "samples_per_chain": spc, "epoch_ms": ms}}))
+
+
+def headline_params():
+    import rbmgpu
+
+    if os.path.exists(CHECKPOINT):
+        return read_rbm(CHECKPOINT, rbmgpu.Model(kitaev6_spec())), "trained checkpoint kitaev6_sym.rbm"
+    return None, "N(0, 0.05^2) re+im (rbm_base.cpp:68-88 initialiser, seed 421390484)"
 
 
 def main():
@@ -261,68 +405,74 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the plain vs learned proposal comparison")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--sampler-only", type=int, default=0, metavar="SPC",
-                    help="only run three sampling epochs with SPC kept samples per chain")
+    ap.add_argument("--no-side", action="store_true", help="headline line only")
+    ap.add_argument("--eager", action="store_true", help="launch the epoch eagerly instead of replaying a graph")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--sampler-only", type=int, default=0, metavar="SPC")
+    ap.add_argument("--workload", default="kitaev6", choices=["kitaev6", "toric4"])
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
     if a.sampler_only:
-        sampler_only(a.sampler_only)
+        sampler_only(a.sampler_only, a.workload)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box here has one GPU, and the "
               "multi-GPU path is a design (chains shard across GPUs), not a run.", file=sys.stderr)
         sys.exit(2)
-    m, g, dt, log, times = gpu_epochs(a.steps, a.warmup)
-    S = WORKLOAD["chains"] * WORKLOAD["samples_per_chain"]
-    ms = 1e3 * dt / a.steps
-    value = S * a.steps / dt
-    # dominant kernel alone: back-to-back launches on the last sample set,
-    # between two HIP events on the context's stream
-    sx_ms = g.time_sx(20)
-    roofline = sx_roofline(g, m, S, sx_ms)
-    times = dict(times)
-    times.pop("unused", None)
-    times["sx_main"] = sx_ms
-    last = log[-1]
-    tau = float(np.mean([l["tau"] for l in log[a.warmup:]]))
+    params, wdesc = headline_params()
+    m, g, main_line = epochs(kitaev6_spec(), 32, a.steps, a.warmup, params=params, graph=not a.eager)
+    roof = sampler_roofline(g, m, 16384, 8, 100)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_line["samples_per_s"],
         "unit": "samples/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": main_line["ms_per_epoch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "toric code 4x4 (N=32), basic RBM alpha=2 (P=2144), weights N(0,0.05^2) re+im, "
-                        "4096 chains x 64 samples, 100 warm-up sweeps, bond_flips 0.5, SR+CG (max 100 "
-                        "iterations, FP64 vectors); one step = one SR epoch",
-            "samples_per_epoch": S,
+            "workload": "Kitaev honeycomb 6x6 (N=72), symmetric RBM alpha=4 (72 symmetries, P=293), "
+                        "16384 chains x 8 samples, 100 warm-up sweeps, bond_flips 0.5, FP32 sampler, SR+CG "
+                        "(max 100 iterations, FP64 vectors); one step = one SR epoch replayed from a HIP graph",
+            "weights": wdesc,
+            "samples_per_epoch": main_line["samples_per_epoch"],
             "parallelism": "single GPU",
         },
-        "effective_samples_per_s": value / (1 + 2 * max(tau, 0.0)),
-        "tau_int": tau,
-        "energy_per_site": last["E"],
-        "energy_stddev_per_site": last["std"],
-        "acceptance": last["acc"],
-        "cg_iterations": last["itn"],
-        "kernel_ms": times,
-        "launch_shape": list(g.launch_shape()),
-        "roofline": roofline,
+        "effective_samples_per_s": main_line["effective_samples_per_s"],
+        "tau_int": main_line["tau_int"],
+        "energy_per_site": main_line["energy_per_site"],
+        "energy_stddev_per_site": main_line["energy_stddev_per_site"],
+        "energy_error_vs_exact_per_site": main_line["energy_per_site"] - E_EXACT_KITAEV6,
+        "acceptance": main_line["acceptance"],
+        "cg_iterations": main_line["cg_iterations"],
+        "graph_replayed": main_line["graph_replayed"],
+        "roofline": roof,
     }
+    if not a.no_side:
+        side = {}
+        _, _, side["kitaev6_weights_1e-4"] = epochs(kitaev6_spec(weights=1e-4), 32, a.steps, a.warmup)
+        m2, g2, side["toric4_fp32"] = epochs(toric4_spec(), 32, a.steps, a.warmup)
+        side["toric4_fp32"]["sx_roofline"] = struct_roofline(g2, m2, m2.n_samples)
+        del g2
+        _, _, side["toric4_fp64"] = epochs(toric4_spec(), 64, max(2, a.steps // 2), 1)
+        _, _, side["kitaev4_pfaffian_fp32"] = epochs(pfaffian_spec(), 32, a.steps, a.warmup)
+        _, _, side["kitaev6_eager"] = epochs(kitaev6_spec(), 32, a.steps, 1, params=params, graph=False)
+        out["side"] = side
     if not a.no_learned:
-        out["plain_vs_learned"] = plain_vs_learned()
+        out["plain_vs_learned"] = plain_vs_learned(params)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        cb = cpu_baseline(params)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = {"samples_per_s": out["value"] / cb["value"],
+                                 "effective_samples_per_s": out["effective_samples_per_s"] / cb["effective_value"]}
+        out["cpu_calibration"] = calibration()
     print(json.dumps(out))
 
 

@@ -120,6 +120,17 @@ int rbmgpu_sample_epoch(rbmgpu_ctx* ctx, uint32_t epoch);
  * local observables, theta and O for these states as if they were sampled */
 int rbmgpu_load_samples(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n);
 
+/* full_sampler::sample (full_sampler.cpp:43-185): the sample set becomes
+ * all 2^N states (N <= 30) in the reference's order -- 2^parallel_bits
+ * Gray-code runs over the upper bits, run b starting at the state b
+ * (sampler.full.n_parallel_bits) -- each weighted by |psi|^2.  Moments, S.x
+ * and the explicit S then use the weighted aggregator semantics
+ * (aggregator.cpp:84-164, 270-305).  Needs the O matrix (keep_o). */
+int rbmgpu_full_epoch(rbmgpu_ctx* ctx, int32_t parallel_bits);
+/* log psi (2^N complex) and the weights |psi|^2 / max |psi|^2 (2^N) of the
+ * last rbmgpu_full_epoch, in its state order; either pointer may be NULL */
+int rbmgpu_full_weights(const rbmgpu_ctx* ctx, double* logpsi, double* w);
+
 /* aggregator::finalize for the registered aggregators of SR
  * (abstract_optimizer.cpp:184-192, SR.cpp:74-79): E statistics, <O>,
  * Var(O), <E O*> and the means of the other observable groups.  Array

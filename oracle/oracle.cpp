@@ -947,6 +947,9 @@ static void aprod(size_t P, size_t S, size_t n_neural, const cd* O, const cd* d,
     double maxd = 0;
     for (size_t p = 0; p < P; p++) maxd = std::max(maxd, diag[p]);
     std::vector<cd> y(S, 0.);
+    /* every y_k and x_p is summed by one thread in a fixed order, so the
+     * result does not depend on the thread count */
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < S; k++) {
         cd acc = 0.;
         for (size_t p = 0; p < P; p++) acc += O[p + k * P] * b[p];
@@ -954,6 +957,7 @@ static void aprod(size_t P, size_t S, size_t n_neural, const cd* O, const cd* d,
     }
     cd db = 0.;
     for (size_t p = 0; p < P; p++) db += d[p] * b[p];
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < P; p++) {
         double reg = diag[p] * (p < n_neural ? r1 : r1 + rd) + maxd * r2;
         cd acc = 0.;
@@ -1473,6 +1477,7 @@ int orc_epoch(orc_model* m, uint64_t seed, uint32_t epoch, int threads, int use_
         double rtol = c.sr_rtol == 0. ? 1e-16 : c.sr_rtol;
         size_t itn;
         double rs;
+        if (threads > 0) omp_set_num_threads(threads);  // the S.x loops of the CG
         cg1(P, S, s.mach.n_rbm, O.data(), ad.result.data(), ad.variance.data(), r1, r2, rd, F.data(),
             s.dw.data(), mi, rtol, itn, rs);
         double mx = 0;

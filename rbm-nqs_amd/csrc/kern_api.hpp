@@ -126,9 +126,10 @@ struct MomArgs {
     int P, n_groups, ld;
     const cplx<R>* O;
     const double* obs;
-    double* part;  // blocks x P x kOPartRow (k_o_partial)
+    double* part;      // blocks x P x kOPartRow (k_o_partial)
+    const double* w;   // per-sample weights (exact enumeration); nullptr: 1
 };
-constexpr int kOPartRow = 7;
+constexpr int kOPartRow = 9;
 
 struct CgStatus {
     double rsold, rs;
@@ -148,6 +149,7 @@ struct SxArgs {
     const double* maxvar;
     double r1, r2, rd;
     const double* reg_dev;  // non-null: r1, r2, rd read from device memory (graph replays)
+    const double* wsx;      // non-null: per-sample weight S w_k / sum w of an enumerated set
     cplx<R>* part;       // nblocks x ld
     double* part2;       // nchunks x P complex
     double* y;           // complex P
@@ -248,6 +250,7 @@ struct SyrkArgs {
     int64_t S, kper;
     int ncol, nvalid, ntile, npairs, ksplit;  // nvalid = 2 P: columns past it read as 0
     R* part;                                  // ksplit x npairs x 64 x 64
+    const double* wsx;                        // non-null: rows scaled by sqrt(S w_k / sum w)
 };
 template <typename R>
 hipError_t launch_syrk(const SyrkArgs<R>& a, hipStream_t st);
@@ -266,9 +269,11 @@ hipError_t launch_dense_mv(const double* Sm, const double* x, double* y, int P, 
 struct Shape {
     int L, K;
 };
+// (32, 9) and (64, 5) fit the 4 x 72 = 288 units of the 6x6 symmetric
+// RBM with alpha = 4 without padding to 512
 constexpr Shape kShapes[] = {{8, 1},  {8, 2},  {8, 4},  {8, 8},  {8, 16}, {16, 2},
                              {16, 4}, {16, 8}, {16, 16}, {32, 2}, {32, 4}, {32, 8},
-                             {32, 16}, {64, 1}, {64, 2}, {64, 4}, {64, 8}};
+                             {32, 9}, {32, 16}, {64, 1}, {64, 2}, {64, 4}, {64, 5}, {64, 8}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <typename R>

@@ -49,8 +49,9 @@ __device__ inline void syrk_stage(const SyrkArgs<R>& a, R (*As)[kGT + 1], R (*Bs
         const int64_t k = kc + s;
         const bool in = k < k1;
         const int64_t row = (in ? k : k0) * a.ncol;
-        As[s][c] = (in && c0 + c < a.nvalid) ? a.X[row + c0 + c] : (R)0;
-        Bs[s][c] = (in && c1 + c < a.nvalid) ? a.X[row + c1 + c] : (R)0;
+        const R f = a.wsx ? (R)sqrt(a.wsx[in ? k : k0]) : (R)1;
+        As[s][c] = (in && c0 + c < a.nvalid) ? f * a.X[row + c0 + c] : (R)0;
+        Bs[s][c] = (in && c1 + c < a.nvalid) ? f * a.X[row + c1 + c] : (R)0;
     }
 }
 

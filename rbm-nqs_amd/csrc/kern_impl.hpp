@@ -846,25 +846,29 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
     // per block of samples: sums of O - K, |O - K|^2 with the shift K = the
     // block's first O (shifted-data variance: no cancellation when O is
-    // nearly constant, e.g. saturated hidden units) and of E conj(O).
-    // part row per (block, p): [K re, K im, sum d re, sum d im, sum |d|^2,
-    // sum E conj(O) re, im]; k_o_final merges the blocks pairwise.
+    // nearly constant, e.g. saturated hidden units) and of E conj(O), each
+    // sample weighted by w (1 without weights).  part row per (block, p):
+    // [K re, K im, sum w d re, sum w d im, sum w |d|^2, sum w E conj(O) re,
+    // im, sum w, sum w^2]; k_o_final merges the blocks pairwise.
     int64_t k0 = (int64_t)blockIdx.x * a.chunk, k1 = k0 + a.chunk;
     if (k1 > a.S) k1 = a.S;
     for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
         const cplx<R> o0 = a.O[k0 * a.ld + p];
         const double kr = o0.re, ki = o0.im;
-        double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0;
+        double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0, sw = 0, sw2 = 0;
         for (int64_t k = k0; k < k1; k++) {
             cplx<R> o = a.O[k * a.ld + p];
             double ore = o.re, oim = o.im;
             double ere = a.obs[k * 2 * a.n_groups], eim = a.obs[k * 2 * a.n_groups + 1];
+            const double w = a.w ? a.w[k] : 1.0;
             const double dr = ore - kr, di = oim - ki;
-            s0 += dr;
-            s1 += di;
-            s2 += dr * dr + di * di;
-            e0 += ere * ore + eim * oim;  // E * conj(O)
-            e1 += eim * ore - ere * oim;
+            s0 += w * dr;
+            s1 += w * di;
+            s2 += w * (dr * dr + di * di);
+            e0 += w * (ere * ore + eim * oim);  // E * conj(O)
+            e1 += w * (eim * ore - ere * oim);
+            sw += w;
+            sw2 += w * w;
         }
         double* q = a.part + ((int64_t)blockIdx.x * a.P + p) * kOPartRow;
         q[0] = kr;
@@ -874,6 +878,8 @@ __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
         q[4] = s2;
         q[5] = e0;
         q[6] = e1;
+        q[7] = sw;
+        q[8] = sw2;
     }
 }
 
@@ -934,6 +940,7 @@ __global__ void __launch_bounds__(256) k_sx_partial(SxArgs<R> a) {
                 if (p < a.ld) dot = dot + v[2 * i] * b[p] + v[2 * i + 1] * b[p + 1];
             }
             dot = group_csum<64>(dot);
+            if (a.wsx) dot = (R)a.wsx[k] * dot;  // weighted sample set (exact enumeration)
 #pragma unroll
             for (int i = 0; i < 2 * ZM; i++) z[i] = z[i] + conj(v[i]) * dot;
         } else {
@@ -947,6 +954,7 @@ __global__ void __launch_bounds__(256) k_sx_partial(SxArgs<R> a) {
                 }
             }
             dot = group_csum<64>(dot);
+            if (a.wsx) dot = (R)a.wsx[k] * dot;
 #pragma unroll
             for (int i = 0; i < ZM; i++) {
                 int p = 2 * lane + 128 * i;

@@ -36,7 +36,7 @@ static hipError_t eval_one(const EvalArgs<R>& a, int block, hipStream_t st) {
 
 #define RBM_SHAPES(X) \
     X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(8, 16) X(16, 2) X(16, 4) X(16, 8) X(16, 16) X(32, 2) \
-    X(32, 4) X(32, 8) X(32, 16) X(64, 1) X(64, 2) X(64, 4) X(64, 8)
+    X(32, 4) X(32, 8) X(32, 9) X(32, 16) X(64, 1) X(64, 2) X(64, 4) X(64, 5) X(64, 8)
 // shapes with a Pfaffian instance: one chain per wavefront
 #define PF_SHAPES(X) X(64, 1) X(64, 2) X(64, 4) X(64, 8)
 

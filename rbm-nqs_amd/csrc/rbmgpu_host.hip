@@ -103,6 +103,12 @@ struct rbmgpu_ctx {
     uint64_t graph_gen = 0;
     std::vector<std::pair<int64_t, hipGraphExec_t>> graphs;  // CG budget -> instantiated epoch
     int64_t train_itn = 0;             // CG iterations of the last device epoch
+    // ---- exact enumeration (rbmgpu_full_epoch): a weighted sample set
+    bool weighted = false;
+    double *d_w = nullptr, *d_wsx = nullptr;  // w_k = |psi_k|^2 / max, S w_k / sum w
+    double* d_lp = nullptr;                    // log psi of the enumerated states
+    double* d_wred = nullptr;                  // block partials of the weight reductions
+    int64_t w_cap = 0;
     // ---- explicit S (cg-single / direct)
     double* d_Smat = nullptr;  // P x P complex, column-major
     void* d_gpart = nullptr;   // SYRK partial tiles
@@ -633,32 +639,38 @@ __global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb,
                                                  unsigned long long* maxvar) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
-    // block b (n_b samples): mean_b = K_b + sum d / n_b, M2_b = sum |d|^2 -
-    // |sum d|^2 / n_b; merged pairwise, M2 = sum_b M2_b + n_b |mean_b -
-    // mean|^2 (Chan, Golub and LeVeque; the reference accumulates Welford
-    // updates, aggregator.cpp:190-231)
-    double sr = 0, si = 0, er = 0, ei = 0;
+    // block b (weight W_b = sum w, n_b samples without weights): mean_b = K_b
+    // + sum w d / W_b, M2_b = sum w |d|^2 - |sum w d|^2 / W_b; merged
+    // pairwise, M2 = sum_b M2_b + W_b |mean_b - mean|^2 (Chan, Golub and
+    // LeVeque; the reference accumulates weighted Welford updates,
+    // aggregator.cpp:190-231), divided by W - sum w^2 / W (aggregator.cpp:141-154)
+    double sr = 0, si = 0, er = 0, ei = 0, W = 0, W2 = 0;
     for (int64_t b = 0; b < nb; b++) {
         const double* q = part + (b * P + p) * kOPartRow;
-        const double n = (double)(b + 1 < nb ? chunk : S - b * chunk);
-        sr += n * q[0] + q[2];
-        si += n * q[1] + q[3];
+        sr += q[7] * q[0] + q[2];
+        si += q[7] * q[1] + q[3];
         er += q[5];
         ei += q[6];
+        W += q[7];
+        W2 += q[8];
     }
-    const double inv = 1.0 / (double)S;
+    const double inv = 1.0 / W;
     const double mr = sr * inv, mi = si * inv;
     double m2 = 0;
     for (int64_t b = 0; b < nb; b++) {
         const double* q = part + (b * P + p) * kOPartRow;
-        const double n = (double)(b + 1 < nb ? chunk : S - b * chunk);
+        const double n = q[7];
+        if (n == 0) continue;
         const double dr = q[2] / n, di = q[3] / n;                 // mean_b - K_b
         const double ur = q[0] + dr - mr, ui = q[1] + di - mi;     // mean_b - mean
         m2 += (q[4] - n * (dr * dr + di * di)) + n * (ur * ur + ui * ui);
     }
+    (void)chunk;
+    (void)S;
     mean[2 * p] = mr;
     mean[2 * p + 1] = mi;
-    const double v = S > 1 ? fmax(m2 / ((double)S - 1.0), 0.0) : 0.0;
+    const double den = W - W2 / W;
+    const double v = den > 0 ? fmax(m2 / den, 0.0) : 0.0;
     var[p] = v;
     eo[2 * p] = er * inv;
     eo[2 * p + 1] = ei * inv;
@@ -686,27 +698,36 @@ __device__ void block_sum2(double& x, double& y) {
     __syncthreads();  // ra / rb may be written again by the next call
 }
 
-// Energy moments (aggregator.cpp:84-131 with binning, unit weights) in two
-// launches.  k_e_part: one block per range of `chunk` samples (a range never
-// straddles a bin): the range's sum of every group, and for the energy
-// (group 0) the sum of squared deviations from the range's own mean.
-// part row: [M2, n, (sum re, sum im) x n_groups].
-__global__ void __launch_bounds__(256) k_e_part(const double* obs, int64_t S, int n_groups, int64_t chunk,
-                                                double* part) {
+// Energy moments (aggregator.cpp:84-154 with binning; weights w of an
+// enumerated set, 1 otherwise) in two launches.  k_e_part: one block per
+// range of `chunk` samples (a range never straddles a bin): the range's
+// weighted sum of every group, and for the energy (group 0) the weighted
+// sum of squared deviations from the range's own weighted mean.
+// part row: [M2, W, (sum w x re, im) x n_groups, sum w^2].
+__global__ void __launch_bounds__(256) k_e_part(const double* obs, const double* w, int64_t S, int n_groups,
+                                                int64_t chunk, double* part) {
     const int G = 2 * n_groups;
     const int64_t k0 = (int64_t)blockIdx.x * chunk, k1 = k0 + chunk < S ? k0 + chunk : S;
-    double* out = part + (int64_t)blockIdx.x * (2 + G);
+    double* out = part + (int64_t)blockIdx.x * (3 + G);
+    double sw = 0, sw2 = 0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        const double wk = w ? w[k] : 1.0;
+        sw += wk;
+        sw2 += wk * wk;
+    }
+    block_sum2(sw, sw2);
     double mr = 0, mi = 0;
     for (int g = 0; g < n_groups; g++) {
         double sr = 0, si = 0;
         for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-            sr += obs[k * G + 2 * g];
-            si += obs[k * G + 2 * g + 1];
+            const double wk = w ? w[k] : 1.0;
+            sr += wk * obs[k * G + 2 * g];
+            si += wk * obs[k * G + 2 * g + 1];
         }
         block_sum2(sr, si);
-        if (g == 0) {
-            mr = sr / (double)(k1 - k0);
-            mi = si / (double)(k1 - k0);
+        if (g == 0 && sw > 0) {
+            mr = sr / sw;
+            mi = si / sw;
         }
         if (threadIdx.x == 0) {
             out[2 + 2 * g] = sr;
@@ -715,26 +736,35 @@ __global__ void __launch_bounds__(256) k_e_part(const double* obs, int64_t S, in
     }
     double v = 0, dummy = 0;
     for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        const double wk = w ? w[k] : 1.0;
         const double dr = obs[k * G] - mr, di = obs[k * G + 1] - mi;
-        v += dr * dr + di * di;
+        v += wk * (dr * dr + di * di);
     }
     block_sum2(v, dummy);
     if (threadIdx.x == 0) {
         out[0] = v;
-        out[1] = (double)(k1 - k0);
+        out[1] = sw;
+        out[2 + G] = sw2;
     }
 }
 
 // k_e_final, one block: the means from the range sums, the energy's sum of
-// squared deviations by the pairwise merge M2 = sum_b (M2_b + n_b |m_b - m|^2)
+// squared deviations by the pairwise merge M2 = sum_b (M2_b + W_b |m_b - m|^2)
 // (Chan, Golub and LeVeque; the reference merges its ranks' partial results
-// the same way, aggregator.cpp:101-116), tau as aggregator.cpp:120-131, and
-// the binned variance from the bin sums
-// (per_bin ranges per bin).  out: [mean re, mean im, var, var_binned, tau,
-// std, ..., group means from 16].
+// the same way, aggregator.cpp:101-116), the unbiased factor W - sum w^2 / W
+// (aggregator.cpp:141-154), tau as aggregator.cpp:120-131 (bin_size in
+// samples), and the binned variance from the bin sums (per_bin ranges per
+// bin).  out: [mean re, mean im, var, var_binned, tau, std, ..., group means
+// from 16].
 __global__ void __launch_bounds__(1024) k_e_final(const double* part, int64_t nblk, int64_t per_bin, int64_t S,
                                                   int n_groups, int64_t n_bins, double* out) {
-    const int PS = 2 + 2 * n_groups;
+    const int PS = 3 + 2 * n_groups;
+    double W = 0, W2 = 0;
+    for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
+        W += part[b * PS + 1];
+        W2 += part[b * PS + PS - 1];
+    }
+    block_sum2(W, W2);
     double mr = 0, mi = 0;
     for (int g = 0; g < n_groups; g++) {
         double sr = 0, si = 0;
@@ -743,8 +773,8 @@ __global__ void __launch_bounds__(1024) k_e_final(const double* part, int64_t nb
             si += part[b * PS + 3 + 2 * g];
         }
         block_sum2(sr, si);
-        sr /= (double)S;
-        si /= (double)S;
+        sr /= W;
+        si /= W;
         if (g == 0) {
             mr = sr;
             mi = si;
@@ -757,6 +787,7 @@ __global__ void __launch_bounds__(1024) k_e_final(const double* part, int64_t nb
     double v = 0, dummy = 0;
     for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
         const double n = part[b * PS + 1];
+        if (n == 0) continue;
         const double dr = part[b * PS + 2] / n - mr, di = part[b * PS + 3] / n - mi;
         v += part[b * PS] + n * (dr * dr + di * di);
     }
@@ -765,21 +796,24 @@ __global__ void __launch_bounds__(1024) k_e_final(const double* part, int64_t nb
     const int64_t bs = n_bins > 1 ? S / n_bins : 0;
     if (n_bins > 1) {
         for (int64_t b = threadIdx.x; b < n_bins; b += blockDim.x) {
-            double br = 0, bi = 0;
+            double br = 0, bi = 0, bw = 0;
             for (int64_t j = 0; j < per_bin; j++) {
                 br += part[(b * per_bin + j) * PS + 2];
                 bi += part[(b * per_bin + j) * PS + 3];
+                bw += part[(b * per_bin + j) * PS + 1];
             }
-            const double dr = br / bs - mr, di = bi / bs - mi;
-            vb += bs * (dr * dr + di * di);
+            if (bw > 0) {
+                const double dr = br / bw - mr, di = bi / bw - mi;
+                vb += bw * (dr * dr + di * di);
+            }
         }
         block_sum2(vb, dummy);
     }
     if (threadIdx.x == 0) {
-        double denom = (double)S - 1.0;  // ptotal - wsum2 / ptotal
+        double denom = W - W2 / W;  // ptotal - wsum2 / ptotal
         double tau = 0;
         if (n_bins > 1 && !(v == 0 && vb == 0)) tau = 0.5 * (fmax(bs * vb / v, 1.0) - 1);
-        double var = fmax(v / denom, 0.0), varb = fmax(vb / denom, 0.0);
+        double var = denom > 0 ? fmax(v / denom, 0.0) : 0.0, varb = denom > 0 ? fmax(vb / denom, 0.0) : 0.0;
         out[0] = mr;
         out[1] = mi;
         out[2] = var;
@@ -1186,6 +1220,7 @@ SxArgs<R> sx_args(rbmgpu_ctx* c, const double* x, double* y, double r1, double r
                   const CgStatus* st) {
     SxArgs<R> a{};
     a.reg_dev = c->reg_dev;
+    a.wsx = c->weighted ? c->d_wsx : nullptr;
     a.S = c->S;
     a.nblocks = c->sx_blocks;
     a.P = (int)c->P;
@@ -1287,14 +1322,15 @@ int compute_moments(rbmgpu_ctx* c) {
     }
     const int64_t per_bin = nb > 1 ? unit / chunk : 0;
     const int64_t nblk = nb > 1 ? nb * per_bin : (c->S + chunk - 1) / chunk;
-    const int64_t pneed = nblk * (2 + 2 * c->n_groups) * (int64_t)sizeof(double);
+    const int64_t pneed = nblk * (3 + 2 * c->n_groups) * (int64_t)sizeof(double);
     if (pneed > c->epart_cap) {
         if (int rc = dalloc(ctx, &c->d_epart, pneed)) return rc;
         c->epart_cap = pneed;
     }
     if (int rc = record(c, 4)) return rc;
-    hipLaunchKernelGGL(k_e_part, dim3((unsigned)nblk), dim3(256), 0, c->stream, c->d_obs, c->S, c->n_groups, chunk,
-                       c->d_epart);
+    const double* wts = c->weighted ? c->d_w : nullptr;
+    hipLaunchKernelGGL(k_e_part, dim3((unsigned)nblk), dim3(256), 0, c->stream, c->d_obs, wts, c->S, c->n_groups,
+                       chunk, c->d_epart);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_e_final, dim3(1), dim3(1024), 0, c->stream, c->d_epart, nblk, per_bin, c->S, c->n_groups,
                        nb, c->d_emom);
@@ -1316,11 +1352,11 @@ int compute_moments(rbmgpu_ctx* c) {
         }
         if (c->prec == RBMGPU_FP32) {
             MomArgs<float> a{c->S, chunk, (int)c->P, c->n_groups, (int)c->ld, (const cplx<float>*)c->d_O, c->d_obs,
-                             c->d_part};
+                             c->d_part, wts};
             HIPCK(launch_o_partial<float>(a, (int)blocks, c->stream));
         } else {
             MomArgs<double> a{c->S, chunk, (int)c->P, c->n_groups, (int)c->ld, (const cplx<double>*)c->d_O, c->d_obs,
-                              c->d_part};
+                              c->d_part, wts};
             HIPCK(launch_o_partial<double>(a, (int)blocks, c->stream));
         }
         if (int rc = zero_async(c, c->d_maxvar, sizeof(double))) return rc;
@@ -1539,6 +1575,7 @@ int build_S_t(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
         c->gpart_cap = need;
     }
     a.part = (R*)c->d_gpart;
+    a.wsx = c->weighted ? c->d_wsx : nullptr;
     HIPCK(launch_syrk<R>(a, c->stream));
     HIPCK(launch_s_from_g<R>(a, (int)c->P, (int)c->P_rbm, c->S, c->d_omean, c->d_ovar, c->d_maxvar, r1, r2, rd, mode,
                              c->d_Smat, c->stream));
@@ -1621,10 +1658,59 @@ int dense_solve(rbmgpu_ctx* c, const std::vector<zd>& A, const std::vector<zd>& 
     return 0;
 }
 
+// ---- exact enumeration (full_sampler.cpp:43-185): all 2^N states in the
+// reference's order (2^bp Gray-code runs over the upper N - bp bits, run b
+// starting from the state b), each weighted by |psi|^2
+__global__ void __launch_bounds__(256) k_gray_states(uint64_t* states, int64_t S, int nw, int bp, int N) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S) return;
+    const int64_t run = k >> (N - bp), i = k & ((((int64_t)1) << (N - bp)) - 1);
+    const uint64_t bits = (uint64_t)run | ((uint64_t)(i ^ (i >> 1)) << bp);
+    states[k * nw] = bits;
+    for (int q = 1; q < nw; q++) states[k * nw + q] = 0;
+}
+
+// per-block max of Re log psi
+__global__ void __launch_bounds__(256) k_lp_max(const double* lp, int64_t S, double* part) {
+    __shared__ double red[4];
+    double m = -INFINITY;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S; k += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, lp[2 * k]);
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// w_k = exp(2 (Re log psi_k - max)) = |psi_k|^2 / max |psi|^2 and the block sums
+__global__ void __launch_bounds__(256) k_weights(const double* lp, int64_t S, const double* pmax, int nb, double* w,
+                                                 double* psum) {
+    double m = -INFINITY;
+    for (int b = 0; b < nb; b++) m = fmax(m, pmax[b]);
+    double s = 0, dummy = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S; k += (int64_t)gridDim.x * blockDim.x) {
+        const double v = exp(2.0 * (lp[2 * k] - m));
+        w[k] = v;
+        s += v;
+    }
+    block_sum2(s, dummy);
+    if (threadIdx.x == 0) psum[blockIdx.x] = s;
+}
+
+// S w_k / sum w: the per-sample factor of the weighted S.x (the O matrix of
+// outer_aggregator_lazy holds sqrt(w) O, aggregator.cpp:270-305)
+__global__ void __launch_bounds__(256) k_wsx(const double* w, int64_t S, const double* psum, int nb, double* wsx) {
+    double W = 0;
+    for (int b = 0; b < nb; b++) W += psum[b];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < S; k += (int64_t)gridDim.x * blockDim.x)
+        wsx[k] = (double)S * w[k] / W;
+}
+
 // one sampling epoch on the stream: all chains, the kept samples' local
 // observables and (materialised path) the O matrix; buffers allocated
 int sample_body(rbmgpu_ctx* c, uint32_t epoch) {
     rbmgpu_ctx* ctx = c;
+    c->weighted = false;
     if (int rc = zero_async(c, c->d_obs, c->n_samples * 2 * c->n_groups * sizeof(double))) return rc;
     if (int rc = record(c, 0)) return rc;
     int rc = 0;
@@ -1959,7 +2045,8 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,
                     c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,
                     c->d_rec_k,  c->d_rec_a,  c->d_lacc,   c->d_lprop, c->d_tr,     c->d_tst,    c->d_tseed,
-                    c->d_tw,     c->d_ratio_term, c->d_term_slot, c->d_params, c->d_par, c->d_stats, c->d_Smat, c->d_gpart};
+                    c->d_tw,     c->d_ratio_term, c->d_term_slot, c->d_params, c->d_par, c->d_stats, c->d_Smat, c->d_gpart, c->d_w, c->d_wsx, c->d_lp,
+                    c->d_wred};
     drop_graphs(c);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -2147,10 +2234,61 @@ int rbmgpu_sample_epoch(rbmgpu_ctx* c, uint32_t epoch) {
     return sample_body(c, epoch);
 }
 
+int rbmgpu_full_epoch(rbmgpu_ctx* c, int32_t parallel_bits) {
+    if (!c) return -1;
+    rbmgpu_ctx* ctx = c;
+    if (c->N > 30) return fail(c, "exact enumeration: at most 30 spins here (2^N samples on the device)");
+    if (c->structured) return fail(c, "exact enumeration needs the O matrix (keep_o)");
+    const int bp = std::max(0, std::min((int)parallel_bits, c->N));
+    const int64_t S = (int64_t)1 << c->N;
+    if (int rc = ensure_samples(c, S)) return rc;
+    const int nb = (int)std::min<int64_t>((S + 255) / 256, 1024);
+    if (S > c->w_cap) {
+        if (int rc = dalloc(ctx, &c->d_w, S * sizeof(double))) return rc;
+        if (int rc = dalloc(ctx, &c->d_wsx, S * sizeof(double))) return rc;
+        if (int rc = dalloc(ctx, &c->d_lp, S * 2 * sizeof(double))) return rc;
+        if (!c->d_wred)
+            if (int rc = dalloc(ctx, &c->d_wred, 2 * 1024 * sizeof(double))) return rc;
+        c->w_cap = S;
+    }
+    hipLaunchKernelGGL(k_gray_states, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, c->stream, c->d_states, S,
+                       c->nw, bp, c->N);
+    HIPCK(hipGetLastError());
+    if (int rc = zero_async(c, c->d_obs, S * 2 * c->n_groups * sizeof(double))) return rc;
+    if (int rc = record(c, 0)) return rc;
+    if (int rc = eval(c, c->d_states, S, c->d_lp, c->d_obs, c->d_T, nullptr, nullptr, 0)) return rc;
+    if (int rc = record(c, 1)) return rc;
+    hipLaunchKernelGGL(k_lp_max, dim3(nb), dim3(256), 0, c->stream, c->d_lp, S, c->d_wred);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_weights, dim3(nb), dim3(256), 0, c->stream, c->d_lp, S, c->d_wred, nb, c->d_w,
+                       c->d_wred + 1024);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_wsx, dim3(nb), dim3(256), 0, c->stream, c->d_w, S, c->d_wred + 1024, nb, c->d_wsx);
+    HIPCK(hipGetLastError());
+    c->S = S;
+    c->have_chains = false;
+    c->weighted = true;
+    int rc = c->prec == RBMGPU_FP32 ? run_build_O<float>(c) : run_build_O<double>(c);
+    if (rc) return rc;
+    if (int rc2 = record(c, 2)) return rc2;
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->moments_valid = false;
+    return 0;
+}
+
+int rbmgpu_full_weights(const rbmgpu_ctx* c, double* logpsi, double* w) {
+    if (!c || !c->weighted) return -1;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -2;
+    if (logpsi && hipMemcpy(logpsi, c->d_lp, c->S * 16, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+    if (w && hipMemcpy(w, c->d_w, c->S * 8, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+    return 0;
+}
+
 int rbmgpu_load_samples(rbmgpu_ctx* c, const uint64_t* states, int64_t n) {
     if (!c || !states || n < 1) return -1;
     rbmgpu_ctx* ctx = c;
     if (int rc = ensure_samples(c, n)) return rc;
+    c->weighted = false;
     HIPCK(hipMemcpyAsync(c->d_states, states, n * c->nw * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
     if (int rc = zero_async(c, c->d_obs, n * 2 * c->n_groups * sizeof(double))) return rc;
     HIPCK(hipEventRecord(c->ev[0], c->stream));

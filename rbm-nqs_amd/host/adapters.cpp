@@ -57,24 +57,10 @@ void gpu_rbm::set_params(const std::vector<cplx>& p) {
     push();
 }
 
-void gpu_rbm::initialize_weights(std::mt19937& rng, double std_dev, double std_dev_imag, double pf_std) {
+void gpu_rbm::initialize_weights(std::mt19937& rng) {
     n_updates_ = 0;
-    // the draw order of rbm_base::initialize_weights, then the Pfaffian
-    std::normal_distribution<double> real_dist{0, std_dev},
-        imag_dist{0, std_dev_imag < 0 ? std_dev : std_dev_imag};
-    const auto& L = setup_.layout();
-    std::vector<cplx> p(n_params_, 0.);
-    const int N = L.n_spins, nvb = L.n_vb, na = L.n_alpha;
-    for (int i = 0; i < nvb; i++) p[i] = cplx(real_dist(rng), imag_dist(rng));
-    for (int i = 0; i < na; i++) {
-        p[nvb + i] = cplx(real_dist(rng), imag_dist(rng));
-        for (int j = 0; j < N; j++) p[nvb + na + j + (int64_t)i * N] = cplx(real_dist(rng), imag_dist(rng));
-    }
-    if (has_pfaffian()) {
-        std::normal_distribution<double> dist{0, pf_std};
-        for (int64_t i = n_neural_; i < n_params_; i++) p[i] = cplx(dist(rng), dist(rng));
-    }
-    set_params(p);
+    // rbm_base::initialize_weights (rbm_base.cpp:58-113), then the Pfaffian
+    set_params(setup_.initial_weights(rng));
 }
 
 const std::vector<cplx>& gpu_rbm::params() {
@@ -151,7 +137,9 @@ namespace sampler {
 
 gpu_metropolis_sampler::gpu_metropolis_sampler(machine::gpu_rbm& rbm, const rbmhost::setup& s)
     : rbm_(rbm),
-      n_samples_(s.n_samples()),
+      n_samples_(s.cfg().sa_type == "full" ? (size_t)1 << rbm.n_visible : s.n_samples()),
+      full_(s.cfg().sa_type == "full"),
+      full_bits_((int)s.cfg().sa_full_n_parallel_bits),
       groups_(s.terms().n_groups),
       learn_hidden_(s.cfg().learn_hidden),
       learn_lr_(s.cfg().learn_lr),
@@ -182,7 +170,8 @@ void gpu_metropolis_sampler::set_epoch_stats(const rbmgpu_epoch_stats& st) {
 
 void gpu_metropolis_sampler::sample() {
     rbmgpu_ctx* c = rbm_.ctx();
-    check(c, rbmgpu_sample_epoch(c, epoch_++), "sample_epoch");
+    if (full_) check(c, rbmgpu_full_epoch(c, full_bits_), "full_epoch");  // full_sampler.cpp:43-185
+    else check(c, rbmgpu_sample_epoch(c, epoch_++), "sample_epoch");
     int64_t P = (int64_t)rbm_.get_n_params();
     a_d_.result.resize(P);
     a_d_.variance.resize(P);
@@ -383,7 +372,7 @@ std::vector<std::complex<double>>& stochastic_reconfiguration::gradient(bool log
 // the plain path of optimize() with the matrix-free CG runs as one device
 // epoch: no plugin, no resample heuristic, no learned proposal to train
 bool stochastic_reconfiguration::fused() const {
-    return !plug_ && !resample_ && !learned_ && solver_.method() == RBMGPU_SOLVE_CG;
+    return !plug_ && !resample_ && !learned_ && !sampler_.is_full() && solver_.method() == RBMGPU_SOLVE_CG;
 }
 
 bool stochastic_reconfiguration::train_step() {

@@ -47,9 +47,9 @@ class gpu_rbm {
     size_t get_n_updates() const { return n_updates_; }
     bool has_pfaffian() const { return n_params_ > n_neural_; }
 
-    // rbm_base.cpp:58-113 (+ pfaffian.cpp:139-155 when present)
-    void initialize_weights(std::mt19937& rng, double std_dev, double std_dev_imag = -1.,
-                            double pf_std = 0.01);
+    // rbm_base.cpp:58-113 (+ pfaffian.cpp:139-155 when present), with the
+    // configuration's rbm.weights / weights_imag / weights_type
+    void initialize_weights(std::mt19937& rng);
     // w -= dw and count the update (rbm_base.cpp:117-143)
     void update_weights(const std::vector<cplx>& dw);
     void update_weights_nc(const std::vector<cplx>& dw);
@@ -115,6 +115,8 @@ class gpu_metropolis_sampler {
     void set_epoch_stats(const rbmgpu_epoch_stats& st);
     void train_proposal();
     double get_acceptance_rate() const { return m_.acceptance; }
+    // sampler.type = full: every state of the basis, weighted by |psi|^2
+    bool is_full() const { return full_; }
     size_t get_n_samples() const { return n_samples_; }
     size_t get_my_n_samples() const { return n_samples_; }
     const operators::aggregator& energy() const { return a_h_; }
@@ -126,6 +128,8 @@ class gpu_metropolis_sampler {
    private:
     machine::gpu_rbm& rbm_;
     size_t n_samples_;
+    bool full_ = false;
+    int full_bits_ = 2;
     uint32_t epoch_ = 0;
     rbmgpu_moments m_{};
     operators::aggregator a_h_, a_d_, a_dh_;

@@ -90,6 +90,39 @@ int special_cases(const ini::config& cfg) {
     return 0;
 }
 
+// --store_state (main.cpp:289-302, full_sampler.cpp:150-160): psi of every
+// basis state, normalised, indexed by the state's number (bit i <=> sigma_i =
+// -1), written as an eigen_fstream matrix (2^N x 1 complex) to <name>.state
+void store_state(machine::gpu_rbm& rbm, const rbmhost::setup& s, const ini::config& cfg) {
+    std::cout << "Storing State..." << std::endl;
+    rbmgpu_ctx* c = rbm.ctx();
+    if (rbmgpu_full_epoch(c, (int32_t)cfg.sa_full_n_parallel_bits) != 0)
+        throw std::runtime_error(rbmgpu_last_error(c));
+    int64_t S = 0;
+    rbmgpu_n_samples(c, &S);
+    std::vector<std::complex<double>> lp(S), vec(S);
+    std::vector<uint64_t> st(S * ((rbm.n_visible + 63) / 64));
+    if (rbmgpu_full_weights(c, (double*)lp.data(), nullptr) != 0 ||
+        rbmgpu_get_samples(c, st.data(), nullptr, nullptr, nullptr) != 0)
+        throw std::runtime_error(rbmgpu_last_error(c));
+    double mx = -INFINITY, tot = 0;
+    for (auto& v : lp) mx = std::max(mx, v.real());
+    const size_t nw = (rbm.n_visible + 63) / 64;
+    for (int64_t k = 0; k < S; k++) {
+        const std::complex<double> psi = std::exp(lp[k] - mx);
+        vec[st[k * nw]] = psi;
+        tot += std::norm(psi);
+    }
+    for (auto& v : vec) v /= std::sqrt(tot);
+    std::ofstream f{cfg.name + ".state", std::ios::binary};
+    const uint64_t rows = (uint64_t)S, cols = 1;
+    f.write((const char*)&rows, 8);
+    f.write((const char*)&cols, 8);
+    f.write((const char*)vec.data(), vec.size() * sizeof(std::complex<double>));
+    std::cout << "State stored to '" << cfg.name << ".state'" << std::endl;
+    (void)s;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -98,7 +131,8 @@ int main(int argc, char** argv) {
     if (rc != 0) return rc;
     try {
         if (special_cases(cfg)) return 0;
-        if (cfg.sa_type != "metropolis") throw std::runtime_error("the GPU backend samples with Metropolis chains");
+        if (cfg.sa_type != "metropolis" && cfg.sa_type != "full")
+            throw std::runtime_error("sampler.type '" + cfg.sa_type + "' is not supported by the GPU backend");
         std::cout << "Starting '" << cfg.name << "'!" << std::endl;
         std::unique_ptr<std::mt19937> rng;
         if (cfg.deterministic) {
@@ -111,9 +145,12 @@ int main(int argc, char** argv) {
         std::cout << "Seed: " << cfg.seed << std::endl;
         rbmhost::setup setup(cfg);
         machine::gpu_rbm rbm(setup);
-        if (cfg.rbm_force || !rbm.load(cfg.name))
-            rbm.initialize_weights(*rng, cfg.rbm_weights, cfg.rbm_weights_imag, cfg.rbm_pfaffian_weights);
+        if (cfg.rbm_force || !rbm.load(cfg.name)) rbm.initialize_weights(*rng);
         rc = 0;
+        if (cfg.store_state) {  // a special case of main.cpp:374-377: store and stop
+            store_state(rbm, setup, cfg);
+            return 0;
+        }
         if (cfg.train) {
             epoch_log log(cfg.name, cfg.rbm_force);
             sampler::gpu_metropolis_sampler sampler(rbm, setup);

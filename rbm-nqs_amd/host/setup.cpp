@@ -173,7 +173,8 @@ rbmgpu_config setup::gpu_config() const {
     // the explicit-S solvers (cg-single, direct and possibly cg-dynamic) read O
     const auto& m = cfg_.opt_sr_method;
     const bool explicit_S = cfg_.opt_type == "SR" && (m == "cg-single" || m == "direct" || m == "cg-dynamic");
-    c.keep_o = (cfg_.gpu_keep_o || explicit_S) ? 1 : 0;
+    // exact enumeration weights the samples, which the structured S.x does not do
+    c.keep_o = (cfg_.gpu_keep_o || explicit_S || cfg_.sa_type == "full") ? 1 : 0;
     c.learn_hidden = cfg_.learn_hidden;
     c.learn_fraction = cfg_.learn_fraction;
     c.cluster_kmax = cluster_kmax_;
@@ -187,7 +188,20 @@ std::vector<std::complex<double>> setup::initial_weights(std::mt19937& rng) cons
     std::normal_distribution<double> real_dist{0, sd}, imag_dist{0, si};
     const int N = layout_.n_spins, nvb = layout_.n_vb, na = layout_.n_alpha;
     std::vector<std::complex<double>> p(layout_.n_params(), 0.);
-    for (int i = 0; i < nvb; i++) p[i] = std::complex<double>(real_dist(rng), imag_dist(rng));
+    if (nvb == N && cfg_.model == "kitaev" && !cfg_.rbm_weights_init_type.empty()) {
+        // honeycomb::initialize_vb (honeycomb.cpp:179-191, rbm_base.cpp:72-75):
+        // no draws for v_bias; "stripy" puts i pi/2 on every site of the unit
+        // cells u with ((u / n_uc_b) % 2 + u % n_uc_b) % 2 == 0
+        if (cfg_.rbm_weights_init_type == "stripy") {
+            const auto& lat = model_->get_lattice();
+            for (int i = 0; i < N; i++) {
+                const size_t u = lat.uc_idx(i);
+                if (((u / lat.n_uc_b) % 2 + u % lat.n_uc_b) % 2 == 0) p[i] = std::complex<double>(0, 1.57079632679);
+            }
+        }
+    } else {
+        for (int i = 0; i < nvb; i++) p[i] = std::complex<double>(real_dist(rng), imag_dist(rng));
+    }
     for (int i = 0; i < na; i++) {
         p[nvb + i] = std::complex<double>(real_dist(rng), imag_dist(rng));
         for (int j = 0; j < N; j++)

@@ -74,7 +74,7 @@ def exported_symbols():
                 "rbmgpu_reduce", "rbmgpu_sr_solve", "rbmgpu_apply_S", "rbmgpu_eval_logpsi",
                 "rbmgpu_psi_ratio", "rbmgpu_get_samples", "rbmgpu_n_samples",
                 "rbmgpu_kernel_times", "rbmgpu_synchronize", "rbmgpu_apply_S_noalloc", "rbmgpu_train_epoch",
-                "rbmgpu_sr_solve_ex", "rbmgpu_build_S"],
+                "rbmgpu_sr_solve_ex", "rbmgpu_build_S", "rbmgpu_full_epoch", "rbmgpu_full_weights"],
         "host": ["rbmhost_create", "rbmhost_destroy", "rbmhost_info", "rbmhost_bonds",
                  "rbmhost_perms", "rbmhost_terms", "rbmhost_exact_energy", "rbmhost_use_evaluation",
                  "rbmhost_set_n_samples", "rbmhost_initial_weights", "rbmhost_gpu_create",
@@ -284,6 +284,17 @@ class Gpu:
 
     def sample_epoch(self, epoch):
         self._chk(libs()[0].rbmgpu_sample_epoch(self.c, C.c_uint32(epoch)))
+
+    def full_epoch(self, parallel_bits=2):
+        """exact enumeration of all 2^N states weighted by |psi|^2 (rbmgpu_full_epoch)"""
+        self._chk(libs()[0].rbmgpu_full_epoch(self.c, C.c_int32(parallel_bits)))
+
+    def full_weights(self):
+        S = self.n_samples()
+        lp = np.zeros(S, dtype=np.complex128)
+        w = np.zeros(S)
+        self._chk(libs()[0].rbmgpu_full_weights(self.c, _p(lp, DP), _p(w, DP)))
+        return lp, w
 
     def load_samples(self, states):
         s = np.ascontiguousarray(states, dtype=np.uint64).reshape(-1, self.n_words)
