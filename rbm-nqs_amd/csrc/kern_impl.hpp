@@ -111,7 +111,11 @@ struct Units {
     // ratio factor cosh(theta + delta) / cosh(theta), kept for accept12's
     // tanh update
     static constexpr bool kMob = sizeof(R) == 4 && K <= 4;
-    cplx<R> kc[K], ks[K], kr[K];
+    // FP32 with more units per lane: the same Mobius step, its cosh / sinh
+    // factors reloaded from the tables at the (less frequent) accepted move
+    // instead of kept in registers from the ratio
+    static constexpr bool kMobR = sizeof(R) == 4 && !kMob;
+    cplx<R> kc[kMob ? K : 1], ks[kMob ? K : 1], kr[kMob ? K : 1];
     // K <= 2, L <= 32 (the Pfaffian shapes have L = 64 and no registers to
     // spare): W entries of the last ratio12 flip set, loaded with its C
     // and S entries for accept12
@@ -236,7 +240,7 @@ struct Units {
     // FP32 (kMob) after a sweep: tanh from theta again (the accepted moves
     // update tanh incrementally, see accept12_w)
     __device__ void resync() {
-        if constexpr (kMob) {
+        if constexpr (kMob || kMobR) {
 #pragma unroll
             for (int u = 0; u < K; u++)
                 if (a[u] >= 0) tn[u] = ctanh(th[u]);
@@ -299,13 +303,33 @@ struct Units {
 #pragma unroll
             for (int u = 0; u < K; u++) {
                 if (a[u] < 0) continue;
-                cplx<R> w0 = W[kidx(d, l.P, s[u], f0) * d.n_alpha + a[u]];
+                const int i0 = kidx(d, l.P, s[u], f0) * d.n_alpha + a[u];
+                cplx<R> w0 = W[i0];
                 th[u] = th[u] - (2 * s0) * w0;
-                if (f1 >= 0) {
-                    cplx<R> w1 = W[kidx(d, l.P, s[u], f1) * d.n_alpha + a[u]];
-                    th[u] = th[u] - (2 * s1) * w1;
+                cplx<R> ch, sh;
+                if constexpr (kMobR) {
+                    ch = l.C[i0];
+                    sh = (-s0) * l.S[i0];
                 }
-                update_tanh(u);
+                if (f1 >= 0) {
+                    const int i1 = kidx(d, l.P, s[u], f1) * d.n_alpha + a[u];
+                    cplx<R> w1 = W[i1];
+                    th[u] = th[u] - (2 * s1) * w1;
+                    if constexpr (kMobR) {
+                        const cplx<R> c1 = l.C[i1], sn1 = (-s1) * l.S[i1];
+                        const cplx<R> nch = ch * c1 + sh * sn1;
+                        sh = sh * c1 + ch * sn1;
+                        ch = nch;
+                    }
+                }
+                if constexpr (kMobR) {
+                    // tanh(theta + delta) = (t cosh delta + sinh delta) / (cosh delta + t sinh delta)
+                    const cplx<R> num = tn[u] * ch + sh, r = ch + tn[u] * sh;
+                    const R inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
+                    tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
+                } else {
+                    update_tanh(u);
+                }
             }
         } else {  // the W entries came with ratio12 of the same flip set
 #pragma unroll
