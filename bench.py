@@ -380,14 +380,17 @@ def sampler_only(spc, workload):
     spec = kitaev6_spec(spc=spc) if workload == "kitaev6" else toric4_spec(spc=spc)
     m = rbmgpu.Model(spec)
     g = m.gpu(precision=32)
-    g.set_params(m.initial_weights(421390484))
+    params = headline_params()[0] if workload == "kitaev6" else None
+    g.set_params(m.initial_weights(421390484) if params is None else params)
     ms = []
     for e in range(3):
         t0 = time.perf_counter()
         g.sample_epoch(epoch=e)
         g.synchronize()
         ms.append(1e3 * (time.perf_counter() - t0))
-    print(json.dumps({"sampler_only": {"workload": workload, "samples_per_chain": spc, "epoch_ms": ms}}))
+    print(json.dumps({"sampler_only": {"workload": workload, "samples_per_chain": spc, "epoch_ms": ms,
+                                       "launch_shape": list(g.launch_shape()), "wave_sampler": g.wave_sampler(),
+                                       "acceptance": g.reduce(want_o=False)["acc"]}}))
 
 
 def headline_params():

@@ -66,7 +66,9 @@ typedef struct rbmgpu_config {
     uint64_t seed;
     /* ---- execution */
     int32_t precision;       /* RBMGPU_FP32 or RBMGPU_FP64 */
-    int32_t lanes_per_chain; /* 0: autotune */
+    int32_t lanes_per_chain; /* 0: autotune; 8/16/32/64: lane-group sampler with that
+                                many lanes per chain; -1: one chain per wavefront
+                                (FP32 only) */
     int32_t keep_o;          /* 1: always materialise the O matrix.  0:
                                 a basic RBM in FP32 uses the structured
                                 S.x (O never formed, rbmgpu_get_samples
@@ -100,8 +102,8 @@ const char* rbmgpu_last_error(const rbmgpu_ctx* ctx);
 /* abstract_machine::get_n_params / get_n_neural_params (abstract_machine.hpp:86-101) */
 int rbmgpu_n_params(const rbmgpu_ctx* ctx, int64_t* n_params, int64_t* n_neural);
 /* execution layout (no counterpart in the reference): [lanes per chain,
- * units per lane, block size, S.x path: 1 structured (no O matrix), 0 O
- * materialised] */
+ * units per lane, block size, flags: bit 0 S.x structured (no O matrix),
+ * bit 1 the one-chain-per-wavefront sampler] */
 int rbmgpu_launch_shape(const rbmgpu_ctx* ctx, int32_t* out4);
 
 /* rbm_base::initialize_weights / load (rbm_base.cpp:58-113, 249-288) */

@@ -27,7 +27,7 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare"]
 
 GPU_SOURCES = ["kern_f32.hip", "kern_f64.hip", "kern_struct.hip", "kern_learn.hip", "kern_eloc.hip",
-               "kern_dense.hip", "rbmgpu_host.hip"]
+               "kern_dense.hip", "kern_wave.hip", "rbmgpu_host.hip"]
 HOST_SOURCES = ["lattice.cpp", "model.cpp", "ini.cpp", "setup.cpp", "optim.cpp", "capi.cpp"]
 DRIVER_SOURCES = ["main.cpp", "adapters.cpp"]
 

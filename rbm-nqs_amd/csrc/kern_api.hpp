@@ -67,6 +67,11 @@ struct SampArgs {
     cplx<R>* Opf;    // S x 4 pf_rows: d ln Pf / d fs per kept sample
     uint32_t* acc;
     uint32_t* prop;
+    // k_sample_wave, symmetric RBM: image pairs of the proposal bonds
+    const uint8_t* pmap;   // n_symm x npb: ordered image pair of bond b under symmetry s
+    const int32_t* pairs;  // npair x 2: (lead, other) site of each image pair
+    int npair;             // 0: bond moves composed from the single-site tables
+    const int16_t* tbond;  // per term: the proposal bond of its two sites, -1
 };
 
 // local observables of stored samples (k_eloc): FP32 basic RBM, terms of
@@ -265,9 +270,11 @@ hipError_t launch_s_from_g(const SyrkArgs<R>& a, int P, int n_neural, int64_t S,
 hipError_t launch_dense_mv(const double* Sm, const double* x, double* y, int P, double* pdot, const CgStatus* st,
                            hipStream_t s);
 
-// supported (lanes per chain, units per lane) shapes
+// supported (lanes per chain, units per lane) shapes; wave = 1: the
+// one-chain-per-wavefront kernel k_sample_wave (L = 64)
 struct Shape {
     int L, K;
+    int wave = 0;
 };
 // (32, 9) and (64, 5) fit the 4 x 72 = 288 units of the 6x6 symmetric
 // RBM with alpha = 4 without padding to 512
@@ -294,6 +301,32 @@ __host__ __device__ inline size_t lds_bytes(const Dims& d) {
 
 template <typename R>
 hipError_t launch_sample(const SampArgs<R>& a, Shape s, int block, hipStream_t st);
+
+// ---- k_sample_wave (FP32, one chain per wavefront, kern_wave.hpp)
+// symmetry rows the lane mapping may touch: K (64 / alpha_n) (the padding
+// units read copies of row 0 and are masked)
+__host__ __device__ inline int wave_ns_pad(const Dims& d, int K) {
+    if (d.n_symm == 1) return 1;
+    const int r = K * (64 / d.n_alpha);
+    return r > d.n_symm ? r : d.n_symm;
+}
+// entries of the single-site tables (basic RBM: 64 padding entries after
+// the last site row for the padding units)
+__host__ __device__ inline int wave_n_entries(const Dims& d) { return d.N * d.n_alpha + (d.n_symm == 1 ? 64 : 0); }
+// [CS | pair CS] float4, [W | pair W | rb] float2, [val?] complex double,
+// [rowptr? k? group? rowbase? | pb] int32, [tbond] int16, [sites? mask?
+// perm | pair map] bytes
+__host__ __device__ inline size_t wave_lds_bytes(const Dims& d, int K, int npair) {
+    const size_t ne = wave_n_entries(d), np = (size_t)npair * d.n_alpha * 2;
+    const int nsp = wave_ns_pad(d, K);
+    size_t b = (ne + np) * 16 + (ne + np + d.N) * 8;
+    if (d.t_lds) b += (size_t)d.n_ent * 16 + 4 * ((size_t)d.n_rowptr + 3 * d.n_terms);
+    b += 8 * (size_t)d.npb + 2 * (size_t)d.n_terms;
+    if (d.t_lds) b += 8 * (size_t)d.n_terms + d.n_ent;
+    if (d.n_symm > 1) b += (size_t)nsp * d.N + (npair ? (size_t)nsp * d.npb : 0);
+    return (b + 15) & ~(size_t)15;
+}
+hipError_t launch_sample_wave(const SampArgs<float>& a, int K, int block, hipStream_t st);
 // the Pfaffian factor runs one chain per wavefront
 inline bool shape_ok_pf(Shape s, int N) { return s.L == 64 && N <= kPfMaxN; }
 template <typename R>

@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <complex>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <random>
 #include <string>
@@ -51,6 +52,11 @@ struct rbmgpu_ctx {
     int32_t *d_ratio_term = nullptr, *d_term_slot = nullptr;
     uint8_t *d_tsites = nullptr, *d_mask = nullptr;
     double* d_val = nullptr;
+    // k_sample_wave (symmetric RBM): ordered image pairs of the proposal bonds
+    int npair = 0;
+    uint8_t* d_pmap = nullptr;   // nsymm x npb
+    int32_t* d_pairs = nullptr;  // npair x 2
+    int16_t* d_tbond = nullptr;  // per term: proposal bond of its two sites, -1
     // samples
     int64_t cap = 0, S = 0;
     uint64_t* d_states = nullptr;
@@ -339,6 +345,10 @@ SampArgs<R> samp_args(rbmgpu_ctx* c, uint32_t epoch, int block, int64_t n_sample
     a.pf_refresh = c->pf_refresh;
     a.acc = c->d_acc;
     a.prop = c->d_prop;
+    a.pmap = c->d_pmap;
+    a.pairs = c->d_pairs;
+    a.npair = c->npair;
+    a.tbond = c->d_tbond;
     return a;
 }
 
@@ -349,12 +359,37 @@ bool eloc_after(const rbmgpu_ctx* c) {
            c->n_ratio <= kElocThreads && c->tk_max <= 4 && c->nw <= 2 && eloc_lds_bytes(c->N, c->H) <= 65536;
 }
 
+// the one-chain-per-wavefront sampler (FP32, no Pfaffian factor, at most
+// 128 sites and 512 hidden units; symmetric RBM: alpha divides 64)
+bool wave_ok(const rbmgpu_ctx* c) {
+    return c->prec == RBMGPU_FP32 && c->pf_rows == 0 && c->nw <= 2 && c->H <= 512 &&
+           (c->nsymm == 1 || 64 % c->na == 0) && (int)c->pb.size() / 2 < 32768;
+}
+// its dims: the local-operator tables in LDS when they fit
+Dims wave_dims(const rbmgpu_ctx* c, int block, int K, int npair) {
+    Dims d = dims(c, block);
+    d.w_lds = 1;
+    d.t_lds = 1;
+    if (wave_lds_bytes(d, K, npair) > c->lds_limit) d.t_lds = 0;
+    return d;
+}
+
 template <typename R>
 int run_sample(rbmgpu_ctx* c, uint32_t epoch, Shape s, int block, int64_t n_samples, int warmup) {
     rbmgpu_ctx* ctx = c;
     SampArgs<R> a = samp_args<R>(c, epoch, block, n_samples, warmup);
     a.epoch_dev = c->epoch_dev;
     a.obs_later = eloc_after(c);
+    if constexpr (sizeof(R) == 4) {
+        if (s.wave) {
+            a.npair = c->nsymm > 1 ? c->npair : 0;
+            a.d = wave_dims(c, block, s.K, a.npair);
+            if (wave_lds_bytes(a.d, s.K, a.npair) > c->lds_limit)
+                return fail(c, "the one-chain-per-wavefront sampler's tables do not fit LDS");
+            HIPCK(launch_sample_wave(a, s.K, block, c->stream));
+            return 0;
+        }
+    }
     HIPCK(launch_sample<R>(a, s, block, c->stream));
     return 0;
 }
@@ -574,17 +609,35 @@ int autotune(rbmgpu_ctx* c) {
         }
         if (best.L && (c->lanes_req == 0 || c->lanes_req == L)) cand.push_back(best);
     }
+    // the one-chain-per-wavefront sampler, at 4, 8 and 16 chains per block
+    // (lanes_req -1: only it)
+    std::vector<int> cand_block(cand.size(), 0);
+    if (wave_ok(c) && (c->lanes_req == 0 || c->lanes_req == -1)) {
+        if (c->lanes_req == -1) {
+            cand.clear();
+            cand_block.clear();
+        }
+        const int K = (c->H + 63) / 64, np = c->nsymm > 1 ? c->npair : 0;
+        for (int b : {256, 512, 1024}) {
+            if (wave_lds_bytes(wave_dims(c, b, K, np), K, np) > c->lds_limit) continue;
+            Shape w{64, K};
+            w.wave = 1;
+            cand.push_back(w);
+            cand_block.push_back(b);
+        }
+    }
     if (cand.empty()) return fail(c, "no launch shape for this number of hidden units");
     if (cand.size() == 1) {
         c->shape = cand[0];
-        c->block = pick_block(c, c->shape.L, c->n_chains);
+        c->block = cand_block[0] ? cand_block[0] : pick_block(c, c->shape.L, c->n_chains);
         return 0;
     }
     // time a short epoch (2 samples per chain, 2 warmup sweeps) per shape
     if (int rc = ensure_samples(c, 2 * (int64_t)c->n_chains)) return rc;
     float best_ms = 1e30f;
-    for (Shape s : cand) {
-        int block = pick_block(c, s.L, c->n_chains);
+    for (size_t ci = 0; ci < cand.size(); ci++) {
+        const Shape s = cand[ci];
+        const int block = cand_block[ci] ? cand_block[ci] : pick_block(c, s.L, c->n_chains);
         float ms = 0;
         for (int rep = 0; rep < 2; rep++) {
             HIPCK(hipEventRecord(c->ev[10], c->stream));
@@ -1398,7 +1451,7 @@ int run_eval(rbmgpu_ctx* c, const uint64_t* d_states, int64_t n, double* logpsi,
     a.pb = c->d_pb;
     a.n_flips = nf;
     for (int i = 0; i < nf; i++) a.flips[i] = flips[i];
-    Shape s = c->shape.L ? c->shape : Shape{0, 0};
+    Shape s = c->shape.L && !c->shape.wave ? c->shape : Shape{0, 0};
     if (!s.L) {
         for (int i = 0; i < kNumShapes; i++)
             if (kShapes[i].L == 64 && shape_fits(c, kShapes[i]) && (!s.L || kShapes[i].K < s.K)) s = kShapes[i];
@@ -1988,6 +2041,51 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (upload(ctx, &c->d_tsites, tsites.data(), tsites.size())) return bad(c->err);
     if (upload(ctx, &c->d_mask, mask.data(), mask.size())) return bad(c->err);
     if (upload(ctx, &c->d_val, val.data(), val.size() * 8)) return bad(c->err);
+    {
+        // k_sample_wave's pair tables: every ordered image (perm_s a, perm_s b)
+        // of a proposal bond (a, b) gets an id (at most 255 of them, else
+        // bond moves are composed from the single-site tables), and each
+        // two-site term that is a proposal bond names it
+        const int npb = (int)c->pb.size() / 2;
+        std::vector<int16_t> tbond(c->n_terms, -1);
+        for (int t = 0; t < c->n_terms; t++) {
+            if (tk[t] != 2) continue;
+            const int x = tsites[8 * t], y = tsites[8 * t + 1];
+            for (int b = 0; b < npb; b++)
+                if ((c->pb[2 * b] == x && c->pb[2 * b + 1] == y) || (c->pb[2 * b] == y && c->pb[2 * b + 1] == x)) {
+                    tbond[t] = (int16_t)b;
+                    break;
+                }
+        }
+        if (tbond.empty()) tbond.push_back(-1);
+        if (upload(ctx, &c->d_tbond, tbond.data(), tbond.size() * 2)) return bad(c->err);
+        if (c->nsymm > 1 && npb > 0) {
+            std::map<std::pair<int, int>, int> ids;
+            std::vector<int32_t> pairs;
+            std::vector<uint8_t> pmap((size_t)c->nsymm * npb);
+            bool fits = true;
+            for (int sy = 0; sy < c->nsymm && fits; sy++)
+                for (int b = 0; b < npb; b++) {
+                    const std::pair<int, int> q{c->perm[sy * c->N + c->pb[2 * b]], c->perm[sy * c->N + c->pb[2 * b + 1]]};
+                    auto it = ids.find(q);
+                    if (it == ids.end()) {
+                        if (ids.size() >= 255) {
+                            fits = false;
+                            break;
+                        }
+                        it = ids.emplace(q, (int)ids.size()).first;
+                        pairs.push_back(q.first);
+                        pairs.push_back(q.second);
+                    }
+                    pmap[(size_t)sy * npb + b] = (uint8_t)it->second;
+                }
+            if (fits) {
+                c->npair = (int)ids.size();
+                if (upload(ctx, &c->d_pmap, pmap.data(), pmap.size())) return bad(c->err);
+                if (upload(ctx, &c->d_pairs, pairs.data(), pairs.size() * 4)) return bad(c->err);
+            }
+        }
+    }
     if (dalloc(ctx, &c->d_acc, c->n_chains * 4) || dalloc(ctx, &c->d_prop, c->n_chains * 4)) return bad(c->err);
     {
         size_t sl = struct_lds_bytes(c->N, c->na);
@@ -2046,7 +2144,7 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,
                     c->d_rec_k,  c->d_rec_a,  c->d_lacc,   c->d_lprop, c->d_tr,     c->d_tst,    c->d_tseed,
                     c->d_tw,     c->d_ratio_term, c->d_term_slot, c->d_params, c->d_par, c->d_stats, c->d_Smat, c->d_gpart, c->d_w, c->d_wsx, c->d_lp,
-                    c->d_wred};
+                    c->d_wred, c->d_pmap, c->d_pairs, c->d_tbond};
     drop_graphs(c);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -2070,7 +2168,7 @@ int rbmgpu_launch_shape(const rbmgpu_ctx* c, int32_t* o) {
     o[0] = c->shape.L;
     o[1] = c->shape.K;
     o[2] = c->block;
-    o[3] = c->structured ? 1 : 0;
+    o[3] = (c->structured ? 1 : 0) | (c->shape.wave ? 2 : 0);
     return 0;
 }
 

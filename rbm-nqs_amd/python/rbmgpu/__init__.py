@@ -261,7 +261,11 @@ class Gpu:
 
     def structured(self):
         """True if S.x runs the structured path (no O matrix)"""
-        return bool(self._layout()[3])
+        return bool(self._layout()[3] & 1)
+
+    def wave_sampler(self):
+        """True if the sampler runs one chain per wavefront (k_sample_wave)"""
+        return bool(self._layout()[3] & 2)
 
     def _layout(self):
         o = np.zeros(4, dtype=np.int32)
