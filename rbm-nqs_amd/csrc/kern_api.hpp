@@ -313,16 +313,16 @@ __host__ __device__ inline int wave_ns_pad(const Dims& d, int K) {
 // entries of the single-site tables (basic RBM: 64 padding entries after
 // the last site row for the padding units)
 __host__ __device__ inline int wave_n_entries(const Dims& d) { return d.N * d.n_alpha + (d.n_symm == 1 ? 64 : 0); }
-// [CS | pair CS] float4, [W | pair W | rb] float2, [val?] complex double,
-// [rowptr? k? group? rowbase? | pb] int32, [tbond] int16, [sites? mask?
-// perm | pair map] bytes
+// [CS | pair CS] float4, [W | pair W | rb | entry factors] float2, [val]
+// complex double, [rowptr k group rowbase pb] int32, [tbond] int16,
+// [sites mask perm pair map] bytes (the local-operator tables always in LDS)
 __host__ __device__ inline size_t wave_lds_bytes(const Dims& d, int K, int npair) {
     const size_t ne = wave_n_entries(d), np = (size_t)npair * d.n_alpha * 2;
     const int nsp = wave_ns_pad(d, K);
-    size_t b = (ne + np) * 16 + (ne + np + d.N) * 8;
-    if (d.t_lds) b += (size_t)d.n_ent * 16 + 4 * ((size_t)d.n_rowptr + 3 * d.n_terms);
+    size_t b = (ne + np) * 16 + (ne + np + d.N + d.n_ent) * 8;
+    b += (size_t)d.n_ent * 16 + 4 * ((size_t)d.n_rowptr + 3 * d.n_terms);
     b += 8 * (size_t)d.npb + 2 * (size_t)d.n_terms;
-    if (d.t_lds) b += 8 * (size_t)d.n_terms + d.n_ent;
+    b += 8 * (size_t)d.n_terms + d.n_ent;
     if (d.n_symm > 1) b += (size_t)nsp * d.N + (npair ? (size_t)nsp * d.npb : 0);
     return (b + 15) & ~(size_t)15;
 }

@@ -39,8 +39,9 @@ struct WaveLds {
     const int32_t* pb;
     const uint8_t* P;   // permutations, rows padded to ns_pad
     const uint8_t* PM;  // pair map [s * npb + b], rows padded to ns_pad
-    Terms T;
+    Terms T;            // local-operator tables (LDS)
     const int16_t* tb;  // per term: the proposal bond of its two sites, -1
+    const float2* fac;  // per entry: matrix element x exp(-2 sum_f sigma_f c_f) of its flipped sites
 };
 
 template <int K>
@@ -93,19 +94,44 @@ __device__ inline WaveLds wave_stage(const SampArgs<float>& a, char* smem) {
     }
     const Terms& T = a.T;
     l.T = T;
-    if (d.t_lds) {
-        l.T.val = lds_copy(at, T.val, 2 * (size_t)d.n_ent);
-        l.T.rowptr = lds_copy(at, T.rowptr, d.n_rowptr);
-        l.T.k = lds_copy(at, T.k, d.n_terms);
-        l.T.group = lds_copy(at, T.group, d.n_terms);
-        l.T.rowbase = lds_copy(at, T.rowbase, d.n_terms);
+    {
+        // the entry factors v_e exp(-2 sum_{f flipped} sigma_f c_f): the row
+        // (term t, local configuration loc) fixes the signs of its sites
+        float2* fac = reinterpret_cast<float2*>(at);
+        for (int r = threadIdx.x; r < d.n_terms * 256; r += blockDim.x) {
+            const int t = r >> 8, loc = r & 255, k = T.k[t];
+            if (loc >> k) continue;
+            const int base = T.rowbase[t] + loc;
+            for (int e = T.rowptr[base]; e < T.rowptr[base + 1]; e++) {
+                const int mask = T.mask[e];
+                float br = 0.f, bi = 0.f;
+                for (int i = 0; i < k; i++)
+                    if ((mask >> i) & 1) {
+                        const float m2 = ((loc >> i) & 1) ? 2.f : -2.f;
+                        const cplx<float> c = a.t.rb[T.sites[t * 8 + i]];
+                        br += m2 * c.re;
+                        bi += m2 * c.im;
+                    }
+                float sn, cs;
+                sincos_f(bi, sn, cs);
+                const float ex = __expf(br);
+                const cplx<float> f = cplx<float>{(float)T.val[2 * e], (float)T.val[2 * e + 1]} *
+                                      cplx<float>{ex * cs, ex * sn};
+                fac[e] = make_float2(f.re, f.im);
+            }
+        }
+        at += (size_t)d.n_ent * 8;
+        l.fac = fac;
     }
+    l.T.val = lds_copy(at, T.val, 2 * (size_t)d.n_ent);
+    l.T.rowptr = lds_copy(at, T.rowptr, d.n_rowptr);
+    l.T.k = lds_copy(at, T.k, d.n_terms);
+    l.T.group = lds_copy(at, T.group, d.n_terms);
+    l.T.rowbase = lds_copy(at, T.rowbase, d.n_terms);
     l.pb = lds_copy(at, a.pb, 2 * (size_t)d.npb);
     l.tb = lds_copy(at, a.tbond, d.n_terms);
-    if (d.t_lds) {
-        l.T.sites = lds_copy(at, T.sites, 8 * (size_t)d.n_terms);
-        l.T.mask = lds_copy(at, T.mask, d.n_ent);
-    }
+    l.T.sites = lds_copy(at, T.sites, 8 * (size_t)d.n_terms);
+    l.T.mask = lds_copy(at, T.mask, d.n_ent);
     l.P = l.PM = reinterpret_cast<const uint8_t*>(at);
     if (d.n_symm > 1) {
         uint8_t* P = reinterpret_cast<uint8_t*>(at);
@@ -162,7 +188,7 @@ __device__ inline int wbit(uint64_t s0, uint64_t s1, int f) {
 template <int K, bool SYM>
 struct WaveUnits {
     cplx<float> th[K], tn[K];
-    int pbase, mbase, aoff, ln;
+    int pbase, mbase, aoff, aoff2, ln, lg;  // lg = log2(alpha_n) (symmetric), aoff2 = 2 aoff
     int ustride, mstride;  // symmetric: byte strides of one unit in perm / pair map
     bool vlast;            // the lane's last unit is a real one
 
@@ -170,7 +196,9 @@ struct WaveUnits {
         ln = ln_;
         if constexpr (SYM) {
             const int na = d.n_alpha, g = 64 / na;
-            aoff = ln % na;
+            lg = __builtin_ctz(na);
+            aoff = ln & (na - 1);
+            aoff2 = 2 * aoff;
             const int sl = ln / na;
             pbase = sl * d.N;
             mbase = sl * d.npb;
@@ -178,18 +206,18 @@ struct WaveUnits {
             mstride = g * d.npb;
             vlast = sl + (K - 1) * g < d.n_symm;
         } else {
-            aoff = pbase = mbase = ustride = mstride = 0;
+            aoff = aoff2 = lg = pbase = mbase = ustride = mstride = 0;
             vlast = ln + 64 * (K - 1) < d.H;
         }
     }
     // single-site entry of unit u for site f
     __device__ int entry(const Dims& d, const WaveLds& l, int u, int f) const {
-        if constexpr (SYM) return (int)l.P[pbase + f + u * ustride] * d.n_alpha + aoff;
+        if constexpr (SYM) return ((int)l.P[pbase + f + u * ustride] << lg) + aoff;
         else return f * d.H + ln + 64 * u;
     }
     // pair-table entry of unit u for proposal bond b
     __device__ int pentry(const Dims& d, const WaveLds& l, int u, int b, int combo) const {
-        return ((int)l.PM[mbase + b + u * mstride] * d.n_alpha + aoff) * 2 + combo;
+        return ((int)l.PM[mbase + b + u * mstride] << (lg + 1)) + aoff2 + combo;
     }
     __device__ cplx<float> hbias(const Dims& d, const cplx<float>* hb, int u) const {
         if constexpr (SYM) return hb[aoff];
@@ -223,11 +251,12 @@ struct WaveUnits {
     }
     // log2 of the lane's prod_u |C + tanh (ns0 S)|^2 for one table entry per
     // unit (single flip, or a pair entry)
-    __device__ float lratio1(const float4* tab, const int (&e)[K], float ns0) const {
+    template <class E>
+    __device__ float lratio1(const float4* tab, E e, float ns0) const {
         float lsum = 0.f, prod = 1.f;
 #pragma unroll
         for (int u = 0; u < K; u++) {
-            const cplx<float> r = factor1(u, tab[e[u]], ns0);
+            const cplx<float> r = factor1(u, tab[e(u)], ns0);
             float n = r.re * r.re + r.im * r.im;
             if (u == K - 1 && !vlast) n = 1.f;
             prod *= n;
@@ -272,12 +301,14 @@ struct WaveUnits {
         const float inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
         tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
     }
-    __device__ void accept1(const float4* tab, const float2* wt, const int (&e)[K], float s0) {
+    template <class E>
+    __device__ void accept1(const float4* tab, const float2* wt, E e, float s0) {
         const float ns0 = -s0, t2 = -2.f * s0;
 #pragma unroll
         for (int u = 0; u < K; u++) {
-            const float4 cs = tab[e[u]];
-            const float2 w = wt[e[u]];
+            const int eu = e(u);
+            const float4 cs = tab[eu];
+            const float2 w = wt[eu];
             const cplx<float> r = factor1(u, cs, ns0);
             th[u] = {fmaf(t2, w.x, th[u].re), fmaf(t2, w.y, th[u].im)};
             mobius(u, {cs.x, cs.y}, {ns0 * cs.z, ns0 * cs.w}, r);
@@ -298,10 +329,11 @@ struct WaveUnits {
 
 // local observables of the chain's current state (local_op.cpp:31-60,
 // summed per group as local_op_chain.cpp:54-60): terms in order; an
-// off-diagonal entry's lane-partial ratio, with the matrix element times
-// the visible-bias factor exp(-2 sum sigma_f c_f) folded into lane 0's
-// partial, waits in one of four slots that are reduced over the wave
-// together; lane 0 writes out[2 g], out[2 g + 1]
+// off-diagonal entry's lane-partial ratio, with the entry's factor (matrix
+// element times the visible-bias factor exp(-2 sum sigma_f c_f), a table
+// built at staging) folded into lane 0's partial, waits in one of four
+// slots that are reduced over the wave together; lane 0 writes out[2 g],
+// out[2 g + 1]
 template <int K, bool SYM>
 __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits<K, SYM>& U, uint64_t st0,
                                 uint64_t st1, int npair, double* out) {
@@ -354,10 +386,9 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
         const int tb = SYM && npair ? rfl((int)l.tb[ti]) : -1;
         for (int e = e0; e < e1; e++) {
             const int mask = rfl((int)T.mask[e]);
-            const double vr = rfl(T.val[2 * e]), vi = rfl(T.val[2 * e + 1]);
             if (mask == 0) {
-                ar += vr;
-                ai += vi;
+                ar += rfl(T.val[2 * e]);
+                ai += rfl(T.val[2 * e + 1]);
                 continue;
             }
             cplx<float> p{1.f, 0.f};
@@ -394,22 +425,7 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
                     if ((u & 3) == 3) renorm(p, pe);
                 }
             }
-            if (U.ln == 0) {
-                float br = 0.f, bi = 0.f;
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-                    if ((mask >> i) & 1) {
-                        const float m2 = wbit(st0, st1, fl[i]) ? 2.f : -2.f;
-                        const float2 c = l.rb[fl[i]];
-                        br += m2 * c.x;
-                        bi += m2 * c.y;
-                    }
-                float sn, cs;
-                sincos_f(bi, sn, cs);
-                const float ex = __expf(br);
-                const cplx<float> f = cplx<float>{(float)vr, (float)vi} * cplx<float>{ex * cs, ex * sn};
-                p = p * f;
-            }
+            if (U.ln == 0) p = p * cplx<float>{l.fac[e].x, l.fac[e].y};
             renorm(p, pe);
 #pragma unroll
             for (int i = 0; i < B; i++)
@@ -492,20 +508,17 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
             const float s0 = wbit(st0, st1, f0) ? -1.f : 1.f;
             float bre = -2.f * s0 * rfl(l.rb[f0].x);
             float lr;
-            int e[K];
+            const int combo = f1 >= 0 && s0 != (wbit(st0, st1, f1) ? -1.f : 1.f) ? 1 : 0;
+            auto e1 = [&](int u) __attribute__((always_inline)) { return U.entry(d, l, u, f0); };
+            auto ep = [&](int u) __attribute__((always_inline)) { return U.pentry(d, l, u, b, combo); };
             if (f1 < 0) {
-#pragma unroll
-                for (int u = 0; u < K; u++) e[u] = U.entry(d, l, u, f0);
-                lr = U.lratio1(l.CS, e, -s0);
+                lr = U.lratio1(l.CS, e1, -s0);
             } else {
                 sw++;
                 const float s1 = wbit(st0, st1, f1) ? -1.f : 1.f;
                 bre += -2.f * s1 * rfl(l.rb[f1].x);
                 if (pairs) {
-                    const int combo = s0 == s1 ? 0 : 1;
-#pragma unroll
-                    for (int u = 0; u < K; u++) e[u] = U.pentry(d, l, u, b, combo);
-                    lr = U.lratio1(l.PCS, e, -s0);
+                    lr = U.lratio1(l.PCS, ep, -s0);
                 } else {
                     lr = U.lratio2(d, l, f0, f1, -s0, -s1);
                 }
@@ -514,9 +527,9 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
             const float lr2 = wave_sum(lr) + 2.8853900817779268f * bre;
             if (lu < lr2) {
                 if (f1 < 0) {
-                    U.accept1(l.CS, l.W, e, s0);
+                    U.accept1(l.CS, l.W, e1, s0);
                 } else if (pairs) {
-                    U.accept1(l.PCS, l.PW, e, s0);
+                    U.accept1(l.PCS, l.PW, ep, s0);
                 } else {
                     const float s1 = wbit(st0, st1, f1) ? -1.f : 1.f;
                     U.accept2(d, l, f0, f1, s0, s1);

@@ -365,12 +365,13 @@ bool wave_ok(const rbmgpu_ctx* c) {
     return c->prec == RBMGPU_FP32 && c->pf_rows == 0 && c->nw <= 2 && c->H <= 512 &&
            (c->nsymm == 1 || 64 % c->na == 0) && (int)c->pb.size() / 2 < 32768;
 }
-// its dims: the local-operator tables in LDS when they fit
+// its dims (all tables in LDS)
 Dims wave_dims(const rbmgpu_ctx* c, int block, int K, int npair) {
+    (void)K;
+    (void)npair;
     Dims d = dims(c, block);
     d.w_lds = 1;
     d.t_lds = 1;
-    if (wave_lds_bytes(d, K, npair) > c->lds_limit) d.t_lds = 0;
     return d;
 }
 
@@ -609,7 +610,7 @@ int autotune(rbmgpu_ctx* c) {
         }
         if (best.L && (c->lanes_req == 0 || c->lanes_req == L)) cand.push_back(best);
     }
-    // the one-chain-per-wavefront sampler, at 4, 8 and 16 chains per block
+    // the one-chain-per-wavefront sampler, at 4 to 16 chains per block
     // (lanes_req -1: only it)
     std::vector<int> cand_block(cand.size(), 0);
     if (wave_ok(c) && (c->lanes_req == 0 || c->lanes_req == -1)) {
@@ -618,7 +619,7 @@ int autotune(rbmgpu_ctx* c) {
             cand_block.clear();
         }
         const int K = (c->H + 63) / 64, np = c->nsymm > 1 ? c->npair : 0;
-        for (int b : {256, 512, 1024}) {
+        for (int b : {256, 384, 512, 640, 1024}) {
             if (wave_lds_bytes(wave_dims(c, b, K, np), K, np) > c->lds_limit) continue;
             Shape w{64, K};
             w.wave = 1;
