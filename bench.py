@@ -207,11 +207,21 @@ def sampler_roofline(g, m, chains, spc, warmup):
     t = float(np.median(ms))
     flops = sampler_flops(m, chains, warmup + spc, spc)
     achieved = flops / (t * 1e-3) / 1e12
-    return {"kernel": "k_sample (Metropolis chains of the epoch with their local energies)", "bound": "mfma",
-            "issue": "FP32 VALU (157.3 TFLOP/s is MI355X's FP32 peak, vector and matrix alike)",
+    return {"kernel": "k_sample_wave (Metropolis chains of the epoch with their local energies)", "bound": "mfma",
+            "issue": "FP32 VALU (157.3 TFLOP/s is MI355X's FP32 peak, vector and matrix alike, counting packed "
+                     "FP32; without packed instructions the VALU peak is 78.6 TFLOP/s)",
             "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None, "flops_per_launch": flops, "launch_ms": t}
+            "frac_of_nonpacked_peak": achieved / FP32_NONPACKED_TFLOPS,
+            "traffic": SAMPLER_TRAFFIC_BYTES, "flops_per_launch": flops, "launch_ms": t}
 
+
+# HBM bytes per sampler launch of the headline workload from two rocprofv3
+# --pmc runs of `bench.py --sampler-only 8` (FETCH_SIZE 721 KiB x2.0, then
+# WRITE_SIZE 299250 KiB x1.0, scaled like the k_struct figure below): the
+# kept states and tanh rows it writes
+SAMPLER_TRAFFIC_BYTES = int((721.19 * 2.0 + 299250.0) * 1024)
+# FP32 FMA peak without packed instructions: 256 CUs x 4 SIMDs x 16 lanes x 2 x 2.4 GHz
+FP32_NONPACKED_TFLOPS = 78.6
 
 # HBM bytes per k_struct launch of the toric 4x4 workload from two rocprofv3
 # --pmc runs (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by

@@ -107,3 +107,56 @@ def test_full_epoch_matches_oracle(name, prec, bp):
     assert np.max(np.abs(y - Sm @ x)) <= stol * np.abs(Sm @ x).max()
     S_got = g.build_S(R1, R2, RD)
     assert np.max(np.abs(S_got - Sm)) <= stol * np.abs(Sm).max()
+
+
+def _pauli_string_expectation(logpsi, N, sites, ops):
+    """<psi|P|psi> / <psi|psi> over all 2^N states for the Pauli string P =
+    ops[0] x ops[1] x ... on `sites` (bit 1 = sigma -1).  Two sigma^y
+    factors make the value independent of the sigma^y and basis sign
+    conventions."""
+    s = np.arange(1 << N)
+    psi = np.exp(logpsi - logpsi.real.max())
+    mask = 0
+    amp = np.ones(1 << N, dtype=np.complex128)
+    for f, op in zip(sites, ops):
+        bit = (s >> int(f)) & 1
+        if op in "xy":
+            mask |= 1 << int(f)
+        if op == "y":
+            amp = amp * np.where(bit == 0, -1j, 1j)
+        if op == "z":
+            amp = amp * np.where(bit == 0, 1.0, -1.0)
+    wpsi = amp * psi[s ^ mask]  # (P psi)(s) = <s|P|s ^ mask> psi(s ^ mask)
+    return (np.conj(psi) * wpsi).sum() / (np.abs(psi) ** 2).sum()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_full_epoch_hexagon_plaquettes(prec):
+    """--evaluate's hexagon groups (main.cpp:645-671): the plaquette
+    operator W_p = sx sy sz sx sy sz on the sites of each hexagon in
+    honeycomb::get_hexagons order (kitaev.cpp:52-64), averaged over the exact
+    |psi|^2 distribution on the GPU, equals <psi|W_p|psi>/<psi|psi> built from
+    the oracle's log psi of all 2^8 states (FP64 1e-9, FP32 1e-4)"""
+    print(f"[gpu-test] hexagon plaquettes prec={prec}", flush=True)
+    spec = ini("kitaev", 2, rbm="symmetry", alpha=2)
+    rng = np.random.default_rng(5)
+    m = rbmgpu.Model(spec)
+    m.use_evaluation(hexagons=True)
+    o = po.Oracle(spec)
+    p = random_params(rng, m.P, 0.4)
+    o.set_params(p)
+    g = m.gpu(precision=prec)
+    g.set_params(p)
+    g.full_epoch(parallel_bits=2)
+    r = g.reduce(want_o=False)
+    hexes = o.hexagons()
+    assert len(r["group_means"]) == 1 + len(hexes) and len(hexes) == 4
+    allst = np.arange(1 << m.N, dtype=np.uint64).reshape(-1, 1)
+    lp = o.logpsi(allst)
+    tol = 1e-9 if prec == 64 else 1e-4
+    for j, hx in enumerate(hexes):
+        want = _pauli_string_expectation(lp, m.N, hx, "xyzxyz")
+        got = r["group_means"][1 + j]
+        assert abs(got - want) <= tol, (j, got, want)
+    # and the energy group is the exact expectation value
+    assert abs(r["group_means"][0] - o.exact_expectation()) <= tol * m.N
