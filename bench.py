@@ -241,6 +241,86 @@ def struct_roofline(g, m, S):
             "launch_ms": sx_ms}
 
 
+ED2_CKPT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rbm-nqs_amd", "examples", "kitaev2_sym_ed.rbm")
+ED2_INI = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rbm-nqs_amd", "examples", "kitaev2_full.ini")
+
+
+def ed2_spec(bond_flips, learn=0, lam=0.5, chains=256, spc=256, full=False):
+    """the reference's Kitaev example (2x2, symmetric RBM alpha = 4) with a
+    Metropolis sampler of chains x spc samples, or exact enumeration"""
+    s = open(ED2_INI).read()
+    if not full:
+        s = s.replace("type = full", "type = metropolis")
+    s = s.replace("n_samples = 1600", f"n_samples = {chains * spc}")
+    s = s.replace("metropolis.n_chains = 4", f"metropolis.n_chains = {chains}")
+    s = s.replace("metropolis.bond_flips = 0.2", f"metropolis.bond_flips = {bond_flips}")
+    if learn:
+        s += f"[learn]\nhidden = {learn}\nfraction = {lam}\n"
+    return s
+
+
+def learned_ab(epochs_=8, train_epochs=20, target=1e-5):
+    """plain Metropolis against the learned cluster proposal on a state where
+    plain moves mix badly: the reference's Kitaev 2x2 example trained to the
+    exact-diagonalisation energy (rbm-nqs_amd/examples/kitaev2_sym_ed.rbm,
+    E/N = -0.8660189 against ED -0.8660254), 256 chains x 256 samples per
+    sampling epoch, 100 warm-up sweeps.  The error of each epoch's E/N is
+    measured against the exact expectation value of the same weights (GPU
+    exact enumeration, FP64); time to a fixed error = sampler time per epoch
+    x (rms error / target)^2."""
+    import rbmgpu
+
+    mf = rbmgpu.Model(ed2_spec(0.2, full=True))
+    params = read_rbm(ED2_CKPT, mf)
+    gf = mf.gpu(precision=64)
+    gf.set_params(params)
+    gf.full_epoch()
+    exact = gf.reduce(want_o=False)["mean"].real / mf.N
+    del gf
+    out = {"workload": "Kitaev 2x2 (N=8) symmetric RBM alpha=4 trained to the ED energy (the reference's "
+                       "examples/kitaev/kitaev.ini run), 256 chains x 256 samples, 100 warm-up sweeps, FP32",
+           "exact_energy_per_site": exact, "target_error_per_site": target}
+
+    def run(name, spec, train=0):
+        m = rbmgpu.Model(spec)
+        g = m.gpu(precision=32)
+        g.set_params(params)
+        tr = None
+        for e in range(train):
+            g.sample_epoch(epoch=1000 + e)
+            tr = g.learn_train(lr=3e-3, beta=0.01, batch=m.n_samples)
+        ms, errs, accs, stds = [], [], [], []
+        for e in range(epochs_):
+            g.sample_epoch(epoch=e)
+            r = g.reduce(want_o=False)
+            ms.append(g.kernel_times()["sample"])
+            errs.append(r["mean"].real / m.N - exact)
+            accs.append(r["acc"])
+            stds.append(r["std"] / m.N)
+        t = float(np.mean(ms[1:]))
+        rms = float(np.sqrt(np.mean(np.square(errs))))
+        rec = {"sampler_ms_per_epoch": t, "rms_error_per_site": rms, "mean_error_per_site": float(np.mean(errs)),
+               "binned_std_per_site": float(np.sqrt(np.mean(np.square(stds)))), "acceptance": float(np.mean(accs)),
+               "ms_to_target_error": t * (rms / target) ** 2}
+        if tr is not None:
+            rec.update({"training_epochs": train, "learned_acceptance": tr["accepted"] / max(tr["proposed"], 1)})
+        out[name] = rec
+
+    run("plain_bond_flips_0.2", ed2_spec(0.2))
+    run("plain_bond_flips_0.5", ed2_spec(0.5))
+    run("plain_single_flips", ed2_spec(0.0))
+    run("learned_untrained", ed2_spec(0.2, learn=32, lam=0.5))
+    run("learned", ed2_spec(0.2, learn=32, lam=0.5), train=train_epochs)
+    # the north star's judge: plain single-flip Metropolis, and the learned
+    # proposal with single flips as its plain fallback
+    run("learned_single_fallback", ed2_spec(0.0, learn=32, lam=0.5), train=train_epochs)
+    best_plain = min(out[k]["ms_to_target_error"] for k in out if k.startswith("plain"))
+    out["learned_speedup_to_target_error_vs_best_plain"] = best_plain / out["learned"]["ms_to_target_error"]
+    out["learned_speedup_to_target_error_vs_single_flips"] = (out["plain_single_flips"]["ms_to_target_error"] /
+                                                               out["learned_single_fallback"]["ms_to_target_error"])
+    return out
+
+
 def plain_vs_learned(params, epochs_=3, train_epochs=5, target=1e-4):
     """headline model at the headline weights: plain Metropolis (single and
     bond flips) against the learned cluster proposal (72 -> 256 -> 72,
@@ -423,7 +503,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--sampler-only", type=int, default=0, metavar="SPC")
     ap.add_argument("--workload", default="kitaev6", choices=["kitaev6", "toric4"])
+    ap.add_argument("--learned-ab", action="store_true", help="only the plain vs learned comparison")
     a = ap.parse_args()
+    if a.learned_ab:
+        print(json.dumps({"plain_vs_learned": learned_ab()}))
+        return
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
@@ -479,7 +563,8 @@ def main():
         _, _, side["kitaev6_eager"] = epochs(kitaev6_spec(), 32, a.steps, 1, params=params, graph=False)
         out["side"] = side
     if not a.no_learned:
-        out["plain_vs_learned"] = plain_vs_learned(params)
+        out["plain_vs_learned"] = learned_ab()
+        out["c3_plain_vs_learned"] = plain_vs_learned(params)
     if not a.no_cpu:
         cb = cpu_baseline(params)
         out["cpu_baseline"] = cb

@@ -12,12 +12,17 @@
 // makes the plain move instead (single site or bond, as the reference);
 // the mixture of the two kernels is a valid Metropolis-Hastings chain.
 //
-// Lockstep blocks.  A block holds 8 chains, one per wavefront; every
-// proposal slot forms the logits of all 8 proposed states with two bf16
-// MFMA GEMMs (v_mfma_f32_16x16x16_bf16), H = S W1^T and Z = relu(H) W2^T.
-// The logits are recomputed from the proposed state each time (no
-// incremental hidden layer), so q is an exact function of the state and
-// the chain is exactly balanced for the network as evaluated.
+// Lockstep blocks.  A block holds 8 chains, one per wavefront.  Whether a
+// slot makes a learned or a plain move is drawn from a Philox stream shared
+// by the block (it does not depend on any chain's state, so each chain's
+// mixture of the two Metropolis-Hastings kernels stays valid); only the
+// learned slots evaluate the network, for all 8 chains at once with two
+// bf16 MFMA GEMMs (v_mfma_f32_16x16x16_bf16), H = S W1^T and Z = relu(H)
+// W2^T, and only they synchronise the block.  A chain's current logits go
+// stale when it accepts a plain move and are recomputed at the next learned
+// slot.  Logits are always computed from the state itself (no incremental
+// hidden layer), so q is an exact function of the state and the chain is
+// exactly balanced for the network as evaluated.
 //
 // bf16 MFMA operand layout (16x16x16): lane l supplies A[l%16][4(l/16)+j]
 // and B[4(l/16)+j][l%16], j < 4; accumulator r of lane l is D[4(l/16)+r][l%16].
@@ -182,14 +187,17 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
     float lse = row_lse(m.Zt + w * Np, N, lane);
 
     // per-slot randomness, drawn ahead by the lanes (lane j: slot sw0 + j):
-    // A = philox(chain, epoch, step, slot), B = philox(.., slot | 2^31)
+    // A = philox(chain, epoch, step, slot), B = philox(.., slot | 2^31), and
+    // the block's learned / plain choice G = philox(2^30 | block, epoch,
+    // step, slot)
     int pcode = 0;
     float pua = 0, pus = 0;
     auto draw = [&](int64_t step, int sw0) {
         uint32_t slot = (uint32_t)(sw0 + lane);
         u4 A = philox((uint32_t)chain, epoch, (uint32_t)step, slot, a.k0, a.k1);
         u4 B = philox((uint32_t)chain, epoch, (uint32_t)step, slot | 0x80000000u, a.k0, a.k1);
-        if (A.x < la.lam_thr) {
+        u4 G = philox(0x40000000u | (uint32_t)blockIdx.x, epoch, (uint32_t)step, slot, a.k0, a.k1);
+        if (G.x < la.lam_thr) {
             pcode = (int)(0x40000000u | ((1u + bounded(B.y, (uint32_t)kmax)) << 16));
         } else {
             int f0, f1 = -1;
@@ -207,6 +215,7 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
     };
 
     uint32_t acc = 0, prop = 0, lacc = 0, lprop = 0;
+    bool stale = false;  // Zc, lse are not those of the current state
     uint64_t rs0 = 0, rs1 = 0;
     int rseed = -1, rk = 0;
     float ra = 0;
@@ -226,6 +235,17 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
             int mask = 0, seed = 0, kk = 0;
             uint64_t fm0 = 0, fm1 = 0;
             if (learned) {
+                // the block's chains whose current logits are stale (an
+                // accepted plain move since their last evaluation)
+                if (__syncthreads_or(stale ? 1 : 0)) {
+                    mlp_put_state(m, Np, N, st, 0, 0, w, lane);
+                    mlp_forward16(m, Hn, Np);
+                    if (stale) {
+                        for (int j = lane; j < Np; j += 64) m.Zc[w * Np + j] = m.Zt[w * Np + j];
+                        lse = row_lse(m.Zt + w * Np, N, lane);
+                        stale = false;
+                    }
+                }
                 // seed i ~ softmax(z(sigma)): inclusive scans of the lanes'
                 // probabilities (sites lane and lane + 64), first crossing of us
                 const float* zc = m.Zc + w * Np;
@@ -277,12 +297,13 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
             for (int t = 0; t < 8; t++)
                 if ((mask >> t) & 1) bre += -2.f * sg[t] * l.rb[fl[t]].re;
             lr2 += 2.8853900817779268f * bre;  // 2 / ln 2
-            // logits of the proposed state (every slot, lockstep)
-            mlp_put_state(m, Np, N, st, fm0, fm1, w, lane);
-            mlp_forward16(m, Hn, Np);
+            // logits of the proposed state (learned slots, lockstep)
             const float* zt = m.Zt + w * Np;
-            float lse2 = row_lse(zt, N, lane);
+            float lse2 = 0.f;
             if (learned) {
+                mlp_put_state(m, Np, N, st, fm0, fm1, w, lane);
+                mlp_forward16(m, Hn, Np);
+                lse2 = row_lse(zt, N, lane);
                 float qf = seed_mass(m, m.Zc + w * Np, lse, seed, kk, kmax);
                 float qr = seed_mass(m, zt, lse2, seed, kk, kmax);
                 lr2 += M<float>::log2(qr) - M<float>::log2(qf);
@@ -297,10 +318,14 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
                 U.accept_mask(d, l, fl, sg, mask);
                 st[0] ^= fm0;
                 st[1] ^= fm1;
-                for (int j = lane; j < Np; j += 64) m.Zc[w * Np + j] = zt[j];
-                lse = lse2;
+                if (learned) {
+                    for (int j = lane; j < Np; j += 64) m.Zc[w * Np + j] = zt[j];
+                    lse = lse2;
+                    lacc++;
+                } else {
+                    stale = true;
+                }
                 acc++;
-                if (learned) lacc++;
             }
         }
         if (step >= a.warmup && (step - a.warmup) % a.step == 0) {
