@@ -56,7 +56,7 @@ typedef struct rbmgpu_config {
                                   group 0 is the Hamiltonian */
     int32_t n_groups;
     /* ---- Metropolis sampler (metropolis_sampler.cpp:41-53) */
-    int32_t n_chains;
+    int32_t n_chains;        /* 0: picked by the autotuner (no counterpart in the reference) */
     int64_t n_samples;
     int32_t n_warmup;
     int32_t step_size;
@@ -101,6 +101,10 @@ void rbmgpu_destroy(rbmgpu_ctx* ctx);
 const char* rbmgpu_last_error(const rbmgpu_ctx* ctx);
 /* abstract_machine::get_n_params / get_n_neural_params (abstract_machine.hpp:86-101) */
 int rbmgpu_n_params(const rbmgpu_ctx* ctx, int64_t* n_params, int64_t* n_neural);
+/* chains the sampler runs (config n_chains = 0: the count the autotuner
+ * picked at the first epoch; no counterpart in the reference) */
+int rbmgpu_n_chains(const rbmgpu_ctx* ctx, int64_t* n);
+
 /* execution layout (no counterpart in the reference): [lanes per chain,
  * units per lane, block size, flags: bit 0 S.x structured (no O matrix),
  * bit 1 the one-chain-per-wavefront sampler] */

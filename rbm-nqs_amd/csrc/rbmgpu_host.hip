@@ -34,6 +34,8 @@ struct rbmgpu_ctx {
     int eloc_bpc = 0;  // resident k_eloc blocks per CU (0: not yet queried)
     // ---- sampler
     int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
+    bool auto_chains = false;  // the autotuner picks n_chains (config n_chains = 0)
+    int chains_cap = 1;        // per-chain buffers hold this many chains
     int64_t n_samples = 0;
     double bond_flips = 0.5;
     uint64_t seed = 0;
@@ -598,6 +600,48 @@ int run_build_O(rbmgpu_ctx* c) {
     return 0;
 }
 
+// automatic chain count (config n_chains = 0): for chain counts 2^k from
+// 1024 to min(n_samples, 65536), time a short epoch (4 warm-up sweeps + 2
+// kept) with the tuned launch shape, and keep the count that minimises the
+// predicted epoch, per-sweep time x (warm-up + ceil(n_samples / chains) x
+// sweeps per sample)
+int tune_chains(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    if (!c->auto_chains) return 0;
+    std::vector<int> cand;
+    for (int64_t k = 1024; k <= c->chains_cap; k *= 2) cand.push_back((int)k);
+    if (cand.empty() || cand.back() != c->chains_cap) cand.push_back(c->chains_cap);
+    if (int rc = ensure_samples(c, 2 * (int64_t)c->chains_cap)) return rc;
+    const int keep = c->n_chains;
+    double best = 1e300;
+    int best_c = keep;
+    for (int nc : cand) {
+        c->n_chains = nc;
+        float ms = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            HIPCK(hipEventRecord(c->ev[10], c->stream));
+            const int64_t ns = 2 * (int64_t)nc;
+            int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, 0xFFFFFFF1u, c->shape, c->block, ns, 4)
+                                            : run_sample<double>(c, 0xFFFFFFF1u, c->shape, c->block, ns, 4);
+            if (rc) {
+                c->n_chains = keep;
+                return rc;
+            }
+            HIPCK(hipEventRecord(c->ev[11], c->stream));
+            HIPCK(hipEventSynchronize(c->ev[11]));
+            HIPCK(hipEventElapsedTime(&ms, c->ev[10], c->ev[11]));
+        }
+        const double sweeps = c->warmup + (double)((c->n_samples + nc - 1) / nc) * c->step;
+        const double pred = ms / (4.0 + 2.0 * c->step) * sweeps;
+        if (pred < best) {
+            best = pred;
+            best_c = nc;
+        }
+    }
+    c->n_chains = best_c;
+    return 0;
+}
+
 int autotune(rbmgpu_ctx* c) {
     rbmgpu_ctx* ctx = c;
     if (c->shape.L) return 0;
@@ -631,7 +675,7 @@ int autotune(rbmgpu_ctx* c) {
     if (cand.size() == 1) {
         c->shape = cand[0];
         c->block = cand_block[0] ? cand_block[0] : pick_block(c, c->shape.L, c->n_chains);
-        return 0;
+        return tune_chains(c);
     }
     // time a short epoch (2 samples per chain, 2 warmup sweeps) per shape
     if (int rc = ensure_samples(c, 2 * (int64_t)c->n_chains)) return rc;
@@ -656,7 +700,7 @@ int autotune(rbmgpu_ctx* c) {
             c->block = block;
         }
     }
-    return 0;
+    return tune_chains(c);
 }
 
 // zero fill by a kernel: unlike hipMemsetAsync it may be captured into a
@@ -1934,6 +1978,13 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
         if (b < 0 || b >= c->N) return bad("basis site out of range");
     // ---- sampler settings
     c->n_chains = cfg->n_chains;
+    if (c->n_chains == 0 && cfg->n_samples > 0) {
+        // automatic chain count: start from min(n_samples, 16384), candidates
+        // up to min(n_samples, 65536)
+        c->auto_chains = true;
+        c->n_chains = (int)std::min<int64_t>(cfg->n_samples, 16384);
+    }
+    c->chains_cap = c->auto_chains ? (int)std::min<int64_t>(cfg->n_samples, 65536) : c->n_chains;
     c->n_samples = cfg->n_samples;
     c->warmup = cfg->n_warmup;
     c->step = cfg->step_size;
@@ -2087,7 +2138,8 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
             }
         }
     }
-    if (dalloc(ctx, &c->d_acc, c->n_chains * 4) || dalloc(ctx, &c->d_prop, c->n_chains * 4)) return bad(c->err);
+    if (dalloc(ctx, &c->d_acc, (size_t)c->chains_cap * 4) || dalloc(ctx, &c->d_prop, (size_t)c->chains_cap * 4))
+        return bad(c->err);
     {
         size_t sl = struct_lds_bytes(c->N, c->na);
         c->structured = !c->keep_o && c->nsymm == 1 && c->nvb == c->N && c->pf_rows == 0 &&
@@ -2099,6 +2151,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     if (cfg->learn_hidden > 0) {  // learned proposal
         if (c->prec != RBMGPU_FP32 || c->pf_rows) return bad("the learned proposal needs FP32 and no Pfaffian factor");
         if (cfg->learn_hidden % 16 || cfg->learn_hidden > 512) return bad("learn_hidden: multiple of 16, <= 512");
+        if (c->auto_chains) return bad("the learned proposal needs a fixed number of chains");
         if (cfg->cluster_kmax < 1 || cfg->cluster_kmax > 8 || !cfg->clusters) return bad("invalid cluster table");
         c->Hn = cfg->learn_hidden;
         c->Np = (c->N + 15) / 16 * 16;
@@ -2161,6 +2214,12 @@ int rbmgpu_n_params(const rbmgpu_ctx* c, int64_t* n, int64_t* nn) {
     if (!c) return -1;
     if (n) *n = c->P;
     if (nn) *nn = c->P_rbm;
+    return 0;
+}
+
+int rbmgpu_n_chains(const rbmgpu_ctx* c, int64_t* n) {
+    if (!c || !n) return -1;
+    *n = c->n_chains;
     return 0;
 }
 

@@ -146,6 +146,7 @@ const std::map<std::string, setter>& table() {
          }},
         {"gpu.lanes_per_chain", [](config& c, const std::string& v) { c.gpu_lanes = std::stoi(v); }},
         {"gpu.keep_o", [](config& c, const std::string& v) { c.gpu_keep_o = boolean(v); }},
+        {"gpu.auto_chains", [](config& c, const std::string& v) { c.gpu_auto_chains = boolean(v); }},
         {"learn.hidden", [](config& c, const std::string& v) { c.learn_hidden = std::stoi(v); }},
         {"learn.fraction", [](config& c, const std::string& v) { c.learn_fraction = std::stod(v); }},
         {"learn.lr", [](config& c, const std::string& v) { c.learn_lr = std::stod(v); }},

@@ -74,6 +74,7 @@ struct config {
     int gpu_lanes = 0;
     // materialise the O matrix even where the structured S.x applies
     bool gpu_keep_o = false;
+    bool gpu_auto_chains = false;  // the autotuner picks the chain count
     // learned cluster proposal (new keys; the reference has none)
     int learn_hidden = 0;          // hidden width of the proposal network, 0: off
     double learn_fraction = 0.5;   // probability of a learned move per slot

@@ -160,7 +160,7 @@ rbmgpu_config setup::gpu_config() const {
     c.term_mat = terms_.mat.data();
     c.term_group = terms_.group.data();
     c.n_groups = terms_.n_groups;
-    c.n_chains = (int32_t)cfg_.sa_metropolis_n_chains;
+    c.n_chains = cfg_.gpu_auto_chains && cfg_.sa_type != "full" ? 0 : (int32_t)cfg_.sa_metropolis_n_chains;
     c.n_samples = (int64_t)n_samples_;
     c.n_warmup = (int32_t)warmup_;
     c.step_size = (int32_t)cfg_.sa_metropolis_n_steps_per_sample;

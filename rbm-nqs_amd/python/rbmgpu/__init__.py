@@ -67,7 +67,7 @@ def exported_symbols():
     """Names the two libraries must export (checked by the CPU tests)."""
     return {
         "gpu": ["rbmgpu_create", "rbmgpu_destroy", "rbmgpu_last_error", "rbmgpu_n_params",
-                "rbmgpu_launch_shape", "rbmgpu_set_params", "rbmgpu_get_params", "rbmgpu_time_sx",
+                "rbmgpu_launch_shape", "rbmgpu_n_chains", "rbmgpu_set_params", "rbmgpu_get_params", "rbmgpu_time_sx",
                 "rbmgpu_learn_train", "rbmgpu_learn_grad", "rbmgpu_learn_weights", "rbmgpu_learn_logits",
                 "rbmgpu_learn_set_fraction",
                 "rbmgpu_update_weights", "rbmgpu_sample_epoch", "rbmgpu_load_samples",
@@ -263,6 +263,12 @@ class Gpu:
         """True if S.x runs the structured path (no O matrix)"""
         return bool(self._layout()[3] & 1)
 
+    def chains(self):
+        """chains the sampler runs (the autotuned count with gpu.auto_chains)"""
+        n = C.c_int64()
+        self._chk(libs()[0].rbmgpu_n_chains(self.c, C.byref(n)))
+        return n.value
+
     def wave_sampler(self):
         """True if the sampler runs one chain per wavefront (k_sample_wave)"""
         return bool(self._layout()[3] & 2)
@@ -393,7 +399,7 @@ class Gpu:
         st = np.zeros((S, self.n_words), dtype=np.uint64) if states else None
         ob = np.zeros((S, self.n_groups), dtype=np.complex128) if obs else None
         Om = np.zeros(S * self.P, dtype=np.complex128) if O else None
-        ca = np.zeros(self.n_chains) if chain_acc else None
+        ca = np.zeros(self.chains()) if chain_acc else None
         self._chk(libs()[0].rbmgpu_get_samples(self.c, _p(st, U64P), _p(ob, DP), _p(Om, DP), _p(ca, DP)))
         if Om is not None:
             Om = Om.reshape(S, self.P).T

@@ -111,3 +111,30 @@ def test_wave_and_group_samplers_accept_alike():
     # 2 x 2048 chains x 28 sweeps x ~48 proposals: binomial error ~1e-3
     assert 0.05 < accs[64] < 0.95
     assert abs(accs[-1] - accs[64]) < 0.01, accs
+
+
+def test_auto_chain_count():
+    """gpu.auto_chains: the autotuner picks the chain count (a power of two
+    from 1024, or the sample count) before the first epoch; the epoch's
+    kept samples keep their local-energy and O parity, every chain reports
+    its acceptance"""
+    spec = ini("kitaev", 6, rbm="symmetry", alpha=4, extra=_smp(65536, 64) + "[gpu]\nauto_chains = true\n")
+    print("[gpu-test] auto chains", flush=True)
+    rng = np.random.default_rng(4)
+    m = rbmgpu.Model(spec + KEEP)
+    o = po.Oracle(spec)
+    p = random_params(rng, m.P, 0.05)
+    o.set_params(p)
+    g = m.gpu(precision=32)
+    g.set_params(p)
+    g.sample_epoch(epoch=0)
+    n = g.chains()
+    print(f"[gpu-test] picked {n} chains", flush=True)
+    assert n in (1024, 2048, 4096, 8192, 16384, 32768, 65536)
+    st, obs, O, ca = g.get_samples(states=True, obs=True, O=True, chain_acc=True)
+    assert ca.shape == (n,) and np.all(ca > 0)
+    sel = rng.choice(st.shape[0], 256, replace=False)
+    e_ref = o.eloc(st[sel])
+    assert np.max(np.abs(obs[sel, 0] - e_ref)) <= 1e-4 * (np.abs(e_ref).max() + 1)
+    O_ref = o.derivative(st[sel])
+    assert np.max(np.abs(O[:, sel] - O_ref)) <= 1e-4 * np.abs(O_ref).max()
