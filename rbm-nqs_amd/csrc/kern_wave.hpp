@@ -188,6 +188,7 @@ __device__ inline int wbit(uint64_t s0, uint64_t s1, int f) {
 template <int K, bool SYM>
 struct WaveUnits {
     cplx<float> th[K], tn[K];
+    cplx<float> rr[K];  // the last proposal's unit factors (the Mobius step's denominators)
     int pbase, mbase, aoff, aoff2, ln, lg;  // lg = log2(alpha_n) (symmetric), aoff2 = 2 aoff
     int ustride, mstride;  // symmetric: byte strides of one unit in perm / pair map
     bool vlast;            // the lane's last unit is a real one
@@ -252,11 +253,12 @@ struct WaveUnits {
     // log2 of the lane's prod_u |C + tanh (ns0 S)|^2 for one table entry per
     // unit (single flip, or a pair entry)
     template <class E>
-    __device__ float lratio1(const float4* tab, E e, float ns0) const {
+    __device__ float lratio1(const float4* tab, E e, float ns0) {
         float lsum = 0.f, prod = 1.f;
 #pragma unroll
         for (int u = 0; u < K; u++) {
             const cplx<float> r = factor1(u, tab[e(u)], ns0);
+            rr[u] = r;
             float n = r.re * r.re + r.im * r.im;
             if (u == K - 1 && !vlast) n = 1.f;
             prod *= n;
@@ -275,13 +277,14 @@ struct WaveUnits {
         ch = c0 * c1 + q0 * q1;
         sh = q0 * c1 + c0 * q1;
     }
-    __device__ float lratio2(const Dims& d, const WaveLds& l, int f0, int f1, float ns0, float ns1) const {
+    __device__ float lratio2(const Dims& d, const WaveLds& l, int f0, int f1, float ns0, float ns1) {
         float lsum = 0.f, prod = 1.f;
 #pragma unroll
         for (int u = 0; u < K; u++) {
             cplx<float> ch, sh;
             compose2(l, entry(d, l, u, f0), entry(d, l, u, f1), ns0, ns1, ch, sh);
             const cplx<float> r = ch + tn[u] * sh;
+            rr[u] = r;
             float n = r.re * r.re + r.im * r.im;
             if (u == K - 1 && !vlast) n = 1.f;
             prod *= n;
@@ -294,8 +297,8 @@ struct WaveUnits {
     }
     // accepted move: theta -= 2 s0 W (rbm_base.cpp:174-192) and the Mobius
     // step tanh(theta + delta) = (t ch + sh) / r with r = ch + t sh, the
-    // proposal's unit factor recomputed from the same operands (resync()
-    // restores tanh(theta) after every sweep)
+    // proposal's unit factor kept in rr (resync() restores tanh(theta) after
+    // every sweep)
     __device__ void mobius(int u, cplx<float> ch, cplx<float> sh, cplx<float> r) {
         const cplx<float> num = tn[u] * ch + sh;
         const float inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
@@ -309,9 +312,8 @@ struct WaveUnits {
             const int eu = e(u);
             const float4 cs = tab[eu];
             const float2 w = wt[eu];
-            const cplx<float> r = factor1(u, cs, ns0);
             th[u] = {fmaf(t2, w.x, th[u].re), fmaf(t2, w.y, th[u].im)};
-            mobius(u, {cs.x, cs.y}, {ns0 * cs.z, ns0 * cs.w}, r);
+            mobius(u, {cs.x, cs.y}, {ns0 * cs.z, ns0 * cs.w}, rr[u]);
         }
     }
     __device__ void accept2(const Dims& d, const WaveLds& l, int f0, int f1, float s0, float s1) {
@@ -322,7 +324,7 @@ struct WaveUnits {
             th[u] = {th[u].re - 2.f * (s0 * w0.x + s1 * w1.x), th[u].im - 2.f * (s0 * w0.y + s1 * w1.y)};
             cplx<float> ch, sh;
             compose2(l, e0, e1, -s0, -s1, ch, sh);
-            mobius(u, ch, sh, ch + tn[u] * sh);
+            mobius(u, ch, sh, rr[u]);
         }
     }
 };
