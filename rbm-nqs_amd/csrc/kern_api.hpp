@@ -342,5 +342,8 @@ hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t 
                      hipEvent_t e1 = nullptr);
 // ZM (P <= 64*ZM) supported by launch_sx
 int sx_max_params();
+// waves per k_sx_partial block for an O row length ld (short rows: one
+// block of 16 waves per CU)
+inline int sx_waves(int64_t ld) { return ld <= 128 * 4 ? 16 : 4; }
 
 }  // namespace rbmgpu

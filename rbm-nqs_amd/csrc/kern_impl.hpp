@@ -912,9 +912,10 @@ __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
 // pair of entries is one 16-byte load in FP32).  Each wavefront streams
 // whole rows k: y_k = O_k . x by a wave reduction, then conj(O_k) y_k is
 // accumulated in lane registers (FP32: the row stays in registers; FP64:
-// the row is re-read from cache).  The four waves of a block add their
-// partials in LDS; k_sx_colsum sums the block partials and finishes the
-// product (regularisation and the <O> term).
+// the row is re-read from cache).  The waves of a block add their partials
+// in LDS; k_sx_colsum sums the block partials and finishes the product
+// (regularisation and the <O> term).  Short rows (ld <= 512) run one block
+// of 16 waves per CU, so the column sums read one partial row per CU.
 template <typename R>
 __device__ inline void load_pair(const cplx<R>* o, int p, cplx<R>& a, cplx<R>& b);
 template <>
@@ -931,8 +932,8 @@ __device__ inline void load_pair<double>(const cplx<double>* o, int p, cplx<doub
     b = {v1.x, v1.y};
 }
 
-template <typename R, int ZM>  // ZM pairs per lane: ld <= 128 * ZM
-__global__ void __launch_bounds__(256) k_sx_partial(SxArgs<R> a) {
+template <typename R, int ZM, int WAVES>  // ZM pairs per lane: ld <= 128 * ZM; WAVES per block
+__global__ void __launch_bounds__(64 * WAVES) k_sx_partial(SxArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     if (a.status && a.status->done) return;
     cplx<R>* b = reinterpret_cast<cplx<R>*>(smem);

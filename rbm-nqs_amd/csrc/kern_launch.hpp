@@ -85,24 +85,25 @@ hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <typename R, int ZM>
+template <typename R, int ZM, int WAVES = 4>
 static hipError_t sx_one(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     size_t lds = 2 * (size_t)a.ld * sizeof(cplx<R>);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sx_partial<R, ZM>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_sx_partial<R, ZM, WAVES>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     if (e0 || e1)
-        hipExtLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), (uint32_t)lds, st, e0, e1, 0u, a);
+        hipExtLaunchKernelGGL((k_sx_partial<R, ZM, WAVES>), dim3(blocks), dim3(64 * WAVES), (uint32_t)lds, st, e0,
+                              e1, 0u, a);
     else
-        hipLaunchKernelGGL((k_sx_partial<R, ZM>), dim3(blocks), dim3(256), lds, st, a);
+        hipLaunchKernelGGL((k_sx_partial<R, ZM, WAVES>), dim3(blocks), dim3(64 * WAVES), lds, st, a);
     return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_sx(const SxArgs<R>& a, int blocks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    if (a.ld <= 128 * 2) return sx_one<R, 2>(a, blocks, st, e0, e1);
-    if (a.ld <= 128 * 4) return sx_one<R, 4>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 2) return sx_one<R, 2, 16>(a, blocks, st, e0, e1);
+    if (a.ld <= 128 * 4) return sx_one<R, 4, 16>(a, blocks, st, e0, e1);
     if (a.ld <= 128 * 8) return sx_one<R, 8>(a, blocks, st, e0, e1);
     if (a.ld <= 128 * 12) return sx_one<R, 12>(a, blocks, st, e0, e1);
     if (a.ld <= 128 * 18) return sx_one<R, 18>(a, blocks, st, e0, e1);

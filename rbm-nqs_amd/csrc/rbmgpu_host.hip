@@ -1286,8 +1286,10 @@ int sx_layout(rbmgpu_ctx* c, int64_t* blocks_out, int64_t* bytes_out) {
         row = c->ld_s;
     } else {
         if (c->ld > sx_max_params()) return fail(c, "too many parameters for the S.x kernel");
-        // about 4 blocks of 4 waves per CU, each wave streaming >= 8 rows
-        blocks = std::min<int64_t>(std::max<int64_t>(c->S / 32, 1), 4 * (int64_t)c->n_cu);
+        // 16 waves per CU (one block of 16, or four of 4 for long rows),
+        // each wave streaming >= 8 rows
+        const int wpb = sx_waves(c->ld);
+        blocks = std::min<int64_t>(std::max<int64_t>(c->S / (8 * wpb), 1), (16 / wpb) * (int64_t)c->n_cu);
         row = c->ld;
     }
     *blocks_out = blocks;

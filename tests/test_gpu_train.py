@@ -8,11 +8,12 @@ Per epoch:
   * graph replay == eager launch, bit for bit (same kernels, same order);
     the captured graph is replayed over several epochs with other eager
     work and synchronisations between the replays;
-  * the host-driven path started from the same parameters samples the same
-    chains (same Philox streams; tables from the same parameters), so its
-    energy equals the device epoch's to 1e-10, and after 6 CG iterations its
-    update w - lr x agrees with the device's to 1e-9 of max |lr x| (F and
-    the warm start differ only by rounding between the two paths).
+  * the host-driven path started from the same parameters and the device's
+    warm start samples the same chains (same Philox streams; tables from the
+    same parameters), so its energy equals the device epoch's to 1e-10, and
+    after 6 CG iterations its update w - lr x agrees with the device's to
+    1e-9 of max |lr x| in FP64 (F and the warm start differ only by
+    rounding between the two paths), 1e-4 in FP32.
 """
 import numpy as np
 import pytest
@@ -61,7 +62,7 @@ def test_train_epoch_graph_eager_and_host_path(name):
     _, A = _ctx(spec, prec, p0, L)  # graph replays
     _, Cg = _ctx(spec, prec, p0, L)  # eager launches
     _, B = _ctx(spec, prec, p0, L)  # host-driven steps
-    xB = np.zeros(m.P, np.complex128)
+    xA = np.zeros(m.P, np.complex128)  # the device's warm start: its previous step
     kw = dict(max_iter=6, rtol=1e-300)
     replays = 0
     for ep in range(4):
@@ -80,19 +81,21 @@ def test_train_epoch_graph_eager_and_host_path(name):
         assert abs(r["mean"] - sa["mean"]) <= 1e-10 * abs(sa["mean"]), (ep, r["mean"], sa["mean"])
         assert abs(r["acc"] - sa["acc"]) <= 1e-12
         F = r["eo"] - r["mean"] * r["o_mean"].conj()
-        x, itn, _ = B.sr_solve(F, xB, R1 * 0.9 ** ep, R2, RD, **kw)
+        # the host path starts its CG from the device's own warm start, the
+        # previous epoch's step recovered from the parameters, so each epoch
+        # is compared on its own (no drift carried over)
+        x, itn, _ = B.sr_solve(F, xA, R1 * 0.9 ** ep, R2, RD, **kw)
         assert itn == 6
-        xB = LR * x
-        pb1 = pa - xB
-        err = np.max(np.abs(pb1 - pa1)) / np.abs(xB).max()
+        pb1 = pa - LR * x
+        err = np.max(np.abs(pb1 - pa1)) / np.abs(LR * x).max()
+        xA = pa - pa1  # the warm start of the next solve is this epoch's step lr x
         print(f"[gpu-test] epoch {ep}: E/N {sa['mean'].real / m.N:.6f}, update error {err:.1e}", flush=True)
         # FP64 and the first epoch (zero warm start on both paths): 1e-9.
-        # FP32 later epochs: the two paths start the CG from warm starts that
-        # differ by rounding (~1e-10); the FP32 S.x re-rounds such inputs to
-        # whole FP32 ulps, and 6 CG steps of the ill-conditioned symmetric
-        # system carry that to 1e-5 .. 1e-4 (measured 5.7e-8 structured,
-        # 3.6e-5 / 1.5e-4 symmetric at epochs 1 / 2): stated bound 1e-3
-        assert err <= (1e-9 if prec == 64 or ep == 0 else 1e-3), (ep, err)
+        # FP32 later epochs: the recovered warm start differs from the
+        # device's by rounding (~1e-12 relative); the FP32 S.x re-rounds such
+        # inputs to whole FP32 ulps now and then, and 6 CG steps of the
+        # ill-conditioned symmetric system amplify that: stated bound 1e-4
+        assert err <= (1e-9 if prec == 64 or ep == 0 else 1e-4), (ep, err)
     assert replays >= 2
 
 
