@@ -6,7 +6,8 @@ as for the lane-group sampler); its chains sample |psi|^2 (energy against
 exact enumeration, 5 sigma) with single flips, composed bond flips (basic
 RBM) and the pair tables of the symmetric RBM; its acceptance rate agrees
 with the lane-group sampler's on the same model.  Covers one and several
-units per lane, padding units (H not a multiple of 64) and 4-site terms.
+units per lane (up to the largest, 8), padding units (H not a multiple of
+64), 4-site terms and a sample count that does not divide by the chains.
 """
 import numpy as np
 import pytest
@@ -37,6 +38,10 @@ SPECS = {
     "toric3_sym_a2": ini("toric", 3, rbm="symmetry", alpha=2, extra=_smp(512, 16)),
     # symmetric RBM alpha = 2 on 32 sites
     "kitaev4_sym_a2": ini("kitaev", 4, rbm="symmetry", alpha=2, extra=_smp(512, 16)),
+    # the largest shape: 512 hidden units, eight units per lane
+    "kitaev4_sym_a16": ini("kitaev", 4, rbm="symmetry", alpha=16, extra=_smp(256, 16)),
+    # ragged: 1003 samples over 16 chains (the remainder goes to chain 0)
+    "kitaev3_ragged": ini("kitaev", 3, extra=_smp(1003, 16)),
 }
 
 
@@ -54,10 +59,12 @@ def _gpu(spec, std, seed=1, lanes=-1, keep_o=True):
 @pytest.mark.parametrize("name", list(SPECS))
 def test_wave_sampler_local_energies_and_O(name):
     print(f"[gpu-test] wave sampler {name}", flush=True)
-    m, o, g = _gpu(SPECS[name], 0.1)
+    # 512 hidden units at std 0.1 concentrate |psi|^2 on a few states
+    m, o, g = _gpu(SPECS[name], 0.03 if name == "kitaev4_sym_a16" else 0.1)
     g.sample_epoch(epoch=0)
     assert g.wave_sampler(), g.launch_shape()
     st, obs, O, _ = g.get_samples(states=True, obs=True, O=True)
+    assert st.shape[0] == m.n_samples
     e_ref = o.eloc(st)
     assert np.max(np.abs(obs[:, 0] - e_ref)) <= 1e-4 * (np.abs(e_ref).max() + 1)
     O_ref = o.derivative(st)
