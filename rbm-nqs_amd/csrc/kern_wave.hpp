@@ -335,11 +335,19 @@ struct WaveUnits {
 // element times the visible-bias factor exp(-2 sum sigma_f c_f), a table
 // built at staging) folded into lane 0's partial, waits in one of four
 // slots that are reduced over the wave together; lane 0 writes out[2 g],
-// out[2 g + 1]
+// out[2 g + 1].
+//
+// Symmetric RBM with pair tables: the lanes first resolve the Hamiltonian's
+// terms in parallel (lane ln: terms ln and ln + 64 -- sites, local
+// configuration, row): diagonal-only rows are summed there, and a row with
+// a single off-diagonal entry that flips a proposal bond gets a code for
+// the pair-table path, which then runs term after term with no dependent
+// table reads in between (the codes come by v_readlane); the other terms
+// take the generic path.
 template <int K, bool SYM>
 __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits<K, SYM>& U, uint64_t st0,
                                 uint64_t st1, int npair, double* out) {
-    constexpr int B = 4;
+    constexpr int B = 4, J = 2;
     const Terms& T = l.T;
     const int nt = T.n_terms;
     cplx<float> bp[B];
@@ -363,6 +371,78 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
         }
         nb = 0;
     };
+    auto push = [&](cplx<float> p, int pe, int e) __attribute__((always_inline)) {
+        if (U.ln == 0) p = p * cplx<float>{l.fac[e].x, l.fac[e].y};
+        renorm(p, pe);
+#pragma unroll
+        for (int i = 0; i < B; i++)
+            if (i == nb) {
+                bp[i] = p;
+                bpe[i] = pe;
+            }
+        if (++nb == B) flush();
+    };
+    // the pair-table product of proposal bond tb (lead spin bit sl)
+    auto pair_ratio = [&](int tb, int combo, int sl, cplx<float>& p, int& pe) __attribute__((always_inline)) {
+        p = {1.f, 0.f};
+        pe = 0;
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            cplx<float> r = U.factor1(u, l.PCS[U.pentry(d, l, u, tb, combo)], sl ? 1.f : -1.f);
+            if (u == K - 1 && !U.vlast) r = {1.f, 0.f};
+            p = p * r;
+            if ((u & 3) == 3) renorm(p, pe);
+        }
+    };
+
+    // ---- walk (codes: -1 generic, -2 summed here, >= 0 pair path)
+    const bool walk = SYM && npair > 0 && nt <= 64 * J && d.n_ent < (1 << 14);
+    int wc[J];
+    double dr = 0, di = 0;
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        wc[j] = -1;
+        const int t = U.ln + 64 * j;
+        if (walk && t < nt && T.group[t] == 0) {
+            const int k = T.k[t];
+            int loc = 0;
+            for (int i = 0; i < k; i++) loc |= wbit(st0, st1, (int)T.sites[t * 8 + i]) << i;
+            const int base = T.rowbase[t] + loc, e0 = T.rowptr[base], e1 = T.rowptr[base + 1];
+            bool diag = true;
+            for (int e = e0; e < e1; e++) diag = diag && T.mask[e] == 0;
+            if (diag) {
+                for (int e = e0; e < e1; e++) {
+                    dr += T.val[2 * e];
+                    di += T.val[2 * e + 1];
+                }
+                wc[j] = -2;
+            } else {
+                const int tb = l.tb[t];
+                if (tb >= 0 && k == 2 && e1 - e0 == 1 && T.mask[e0] == 3) {
+                    const int sl = wbit(st0, st1, l.pb[2 * tb]), so = wbit(st0, st1, l.pb[2 * tb + 1]);
+                    wc[j] = tb | ((sl == so ? 0 : 1) << 15) | (sl << 16) | (e0 << 17);
+                }
+            }
+        }
+    }
+    auto code = [&](int t) __attribute__((always_inline)) {
+        return t < 64 ? __builtin_amdgcn_readlane(wc[0], t) : __builtin_amdgcn_readlane(wc[1], t - 64);
+    };
+    if (walk) {
+        ar = group_sum<64>(dr);
+        ai = group_sum<64>(di);
+        for (int t = 0; t < nt; t++) {
+            const int c = code(t);
+            if (c < 0) continue;
+            cplx<float> p;
+            int pe;
+            pair_ratio(c & 0x7fff, (c >> 15) & 1, (c >> 16) & 1, p, pe);
+            push(p, pe, c >> 17);
+        }
+        flush();
+    }
+
+    // ---- the other terms, in order
     int g = nt > 0 ? rfl(T.group[0]) : 0;
     for (int ti = 0; ti < nt; ti++) {
         const int gt = rfl(T.group[ti]);
@@ -375,6 +455,7 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
             ar = ai = 0;
             g = gt;
         }
+        if (walk && code(ti) != -1) continue;
         const int k = rfl(T.k[ti]);
         int fl[8];
         int loc = 0;
@@ -400,14 +481,7 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
                 const int lead = l.pb[2 * tb];
                 const int other = lead == fl[0] ? fl[1] : fl[0];
                 const int sl = wbit(st0, st1, lead), so = wbit(st0, st1, other);
-                const int combo = sl == so ? 0 : 1;
-#pragma unroll
-                for (int u = 0; u < K; u++) {
-                    cplx<float> r = U.factor1(u, l.PCS[U.pentry(d, l, u, tb, combo)], sl ? 1.f : -1.f);
-                    if (u == K - 1 && !U.vlast) r = {1.f, 0.f};
-                    p = p * r;
-                    if ((u & 3) == 3) renorm(p, pe);
-                }
+                pair_ratio(tb, sl == so ? 0 : 1, sl, p, pe);
             } else {
 #pragma unroll
                 for (int u = 0; u < K; u++) {
@@ -427,15 +501,7 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
                     if ((u & 3) == 3) renorm(p, pe);
                 }
             }
-            if (U.ln == 0) p = p * cplx<float>{l.fac[e].x, l.fac[e].y};
-            renorm(p, pe);
-#pragma unroll
-            for (int i = 0; i < B; i++)
-                if (i == nb) {
-                    bp[i] = p;
-                    bpe[i] = pe;
-                }
-            if (++nb == B) flush();
+            push(p, pe, e);
         }
     }
     flush();
