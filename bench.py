@@ -170,6 +170,7 @@ def epochs(spec, precision, steps, warmup, params=None, graph=True, seed=4213904
            "energy_per_site": timed[-1]["mean"].real / m.N, "energy_stddev_per_site": timed[-1]["std"] / m.N,
            "acceptance": float(np.mean([s["acc"] for s in timed])), "cg_iterations": timed[-1]["itn"],
            "graph_replayed": all(s["graph"] for s in timed) if graph else False,
+           "cg_on_explicit_S": bool(timed[-1]["explicit_s"]),
            "launch_shape": list(g.launch_shape()), "structured_sx": g.structured()}
     return m, g, out
 
@@ -259,7 +260,7 @@ def ed2_spec(bond_flips, learn=0, lam=0.5, chains=256, spc=256, full=False):
     return s
 
 
-def learned_ab(epochs_=8, train_epochs=20, target=1e-5):
+def learned_ab(epochs_=8, train_epochs=20, target=1e-5, chains=256, spc=256):
     """plain Metropolis against the learned cluster proposal on a state where
     plain moves mix badly: the reference's Kitaev 2x2 example trained to the
     exact-diagonalisation energy (rbm-nqs_amd/examples/kitaev2_sym_ed.rbm,
@@ -272,13 +273,14 @@ def learned_ab(epochs_=8, train_epochs=20, target=1e-5):
 
     mf = rbmgpu.Model(ed2_spec(0.2, full=True))
     params = read_rbm(ED2_CKPT, mf)
+    spec = lambda bf, **kw: ed2_spec(bf, chains=chains, spc=spc, **kw)  # noqa: E731
     gf = mf.gpu(precision=64)
     gf.set_params(params)
     gf.full_epoch()
     exact = gf.reduce(want_o=False)["mean"].real / mf.N
     del gf
     out = {"workload": "Kitaev 2x2 (N=8) symmetric RBM alpha=4 trained to the ED energy (the reference's "
-                       "examples/kitaev/kitaev.ini run), 256 chains x 256 samples, 100 warm-up sweeps, FP32",
+                       f"examples/kitaev/kitaev.ini run), {chains} chains x {spc} samples, 100 warm-up sweeps, FP32",
            "exact_energy_per_site": exact, "target_error_per_site": target}
 
     def run(name, spec, train=0):
@@ -306,14 +308,14 @@ def learned_ab(epochs_=8, train_epochs=20, target=1e-5):
             rec.update({"training_epochs": train, "learned_acceptance": tr["accepted"] / max(tr["proposed"], 1)})
         out[name] = rec
 
-    run("plain_bond_flips_0.2", ed2_spec(0.2))
-    run("plain_bond_flips_0.5", ed2_spec(0.5))
-    run("plain_single_flips", ed2_spec(0.0))
-    run("learned_untrained", ed2_spec(0.2, learn=32, lam=0.5))
-    run("learned", ed2_spec(0.2, learn=32, lam=0.5), train=train_epochs)
+    run("plain_bond_flips_0.2", spec(0.2))
+    run("plain_bond_flips_0.5", spec(0.5))
+    run("plain_single_flips", spec(0.0))
+    run("learned_untrained", spec(0.2, learn=32, lam=0.5))
+    run("learned", spec(0.2, learn=32, lam=0.5), train=train_epochs)
     # the north star's judge: plain single-flip Metropolis, and the learned
     # proposal with single flips as its plain fallback
-    run("learned_single_fallback", ed2_spec(0.0, learn=32, lam=0.5), train=train_epochs)
+    run("learned_single_fallback", spec(0.0, learn=32, lam=0.5), train=train_epochs)
     best_plain = min(out[k]["ms_to_target_error"] for k in out if k.startswith("plain"))
     out["learned_speedup_to_target_error_vs_best_plain"] = best_plain / out["learned"]["ms_to_target_error"]
     out["learned_speedup_to_target_error_vs_single_flips"] = (out["plain_single_flips"]["ms_to_target_error"] /
@@ -507,6 +509,7 @@ def main():
     a = ap.parse_args()
     if a.learned_ab:
         print(json.dumps({"plain_vs_learned": learned_ab()}))
+        print(json.dumps({"plain_vs_learned_4096x64": learned_ab(chains=4096, spc=64)}))
         return
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
@@ -537,7 +540,9 @@ def main():
         "config": {
             "workload": "Kitaev honeycomb 6x6 (N=72), symmetric RBM alpha=4 (72 symmetries, P=293), "
                         "16384 chains x 8 samples, 100 warm-up sweeps, bond_flips 0.5, FP32 sampler, SR+CG "
-                        "(max 100 iterations, FP64 vectors); one step = one SR epoch replayed from a HIP graph",
+                        "(max 100 iterations, FP64 vectors; its mat-vec on the explicit S, built once per epoch "
+                        "by an MFMA SYRK, when the cost model finds that cheaper); one step = one SR epoch "
+                        "replayed from a HIP graph",
             "weights": wdesc,
             "samples_per_epoch": main_line["samples_per_epoch"],
             "parallelism": "single GPU",
@@ -549,6 +554,7 @@ def main():
         "energy_error_vs_exact_per_site": main_line["energy_per_site"] - E_EXACT_KITAEV6,
         "acceptance": main_line["acceptance"],
         "cg_iterations": main_line["cg_iterations"],
+        "cg_on_explicit_S": main_line["cg_on_explicit_S"],
         "graph_replayed": main_line["graph_replayed"],
         "roofline": roof,
     }

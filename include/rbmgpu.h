@@ -83,6 +83,10 @@ typedef struct rbmgpu_config {
     const int32_t* clusters;  /* N x kmax: the lattice loop through site i,
                                  starting at i */
     uint64_t learn_seed;      /* initial network weights */
+    /* ---- SR epoch (rbmgpu_train_epoch): the CG's matrix-vector product
+     * on the explicit S (1), matrix-free (0), or whichever the cost model
+     * finds cheaper (-1, cg_solver.cpp:31-46) */
+    int32_t sr_explicit_s;
 } rbmgpu_config;
 
 typedef struct rbmgpu_moments {
@@ -177,6 +181,9 @@ typedef struct rbmgpu_epoch_stats {
     int32_t updated;      /* 0: the guard refused the step (driver rc 55) */
     int32_t graph_replayed;
     double ms;            /* host wall time of the call */
+    int32_t explicit_s;   /* 1: the CG ran on the explicit S (cheaper than
+                             matrix-free for this sample set, cg_solver.cpp:31-46) */
+    int32_t pad_;
 } rbmgpu_epoch_stats;
 int rbmgpu_train_epoch(rbmgpu_ctx* ctx, uint32_t epoch, const rbmgpu_sr_params* p, rbmgpu_epoch_stats* out);
 

@@ -256,15 +256,18 @@ struct SyrkArgs {
     int ncol, nvalid, ntile, npairs, ksplit;  // nvalid = 2 P: columns past it read as 0
     R* part;                                  // ksplit x npairs x 64 x 64
     const double* wsx;                        // non-null: rows scaled by sqrt(S w_k / sum w)
+    const double* mean;                       // non-null: columns centred by <O> (2 P doubles) before the
+                                              // products, so S needs no <O><O>^H subtraction
 };
 template <typename R>
 hipError_t launch_syrk(const SyrkArgs<R>& a, hipStream_t st);
 // S (P x P complex, column-major) from the partials: mode 0 the CG
-// regularisation, mode 1 the direct solver's
+// regularisation, mode 1 the direct solver's; reg_dev non-null: r1, r2, rd
+// read from device memory instead
 template <typename R>
 hipError_t launch_s_from_g(const SyrkArgs<R>& a, int P, int n_neural, int64_t S, const double* mean,
-                           const double* var, const double* maxvar, double r1, double r2, double rd, int mode,
-                           double* Sm, hipStream_t st);
+                           const double* var, const double* maxvar, double r1, double r2, double rd,
+                           const double* reg_dev, int mode, double* Sm, hipStream_t st);
 // y = S x of the Hermitian S, per-block partials of x^H y in pdot
 // (ceil(P / 4) blocks); returns at once once st->done
 hipError_t launch_dense_mv(const double* Sm, const double* x, double* y, int P, double* pdot, const CgStatus* st,

@@ -12,7 +12,10 @@
 // MFMA (each of the 4 waves owns a 32 x 32 quarter of the tile as 2 x 2
 // v_mfma_f32_16x16x4_f32 tiles), FP64 by VALU FMAs (4 x 4 per thread);
 // each K slice writes its own partial tile (no atomics, a fixed summation
-// order).  k_s_from_g sums the slices in FP64 and forms S.
+// order).  k_s_from_g sums the slices in FP64 and forms S.  The columns are
+// centred by <O> while they are staged, so the products form the
+// covariance directly: S = <O^H O> - <O>^H <O> would cancel catastrophically
+// in FP32 where |<O>|^2 >> Var(O) (saturated units of converged states).
 //
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; accumulator r of lane l is D[4(l/16)+r][l%16].
@@ -50,8 +53,10 @@ __device__ inline void syrk_stage(const SyrkArgs<R>& a, R (*As)[kGT + 1], R (*Bs
         const bool in = k < k1;
         const int64_t row = (in ? k : k0) * a.ncol;
         const R f = a.wsx ? (R)sqrt(a.wsx[in ? k : k0]) : (R)1;
-        As[s][c] = (in && c0 + c < a.nvalid) ? f * a.X[row + c0 + c] : (R)0;
-        Bs[s][c] = (in && c1 + c < a.nvalid) ? f * a.X[row + c1 + c] : (R)0;
+        const bool va = in && c0 + c < a.nvalid, vb = in && c1 + c < a.nvalid;
+        const R ma = a.mean && va ? (R)a.mean[c0 + c] : (R)0, mb = a.mean && vb ? (R)a.mean[c1 + c] : (R)0;
+        As[s][c] = va ? f * (a.X[row + c0 + c] - ma) : (R)0;
+        Bs[s][c] = vb ? f * (a.X[row + c1 + c] - mb) : (R)0;
     }
 }
 
@@ -159,16 +164,23 @@ __device__ inline double g_at(const SyrkArgs<R>& a, int u, int v) {
 template <typename R>
 __global__ void __launch_bounds__(256) k_s_from_g(SyrkArgs<R> a, int P, int n_neural, int64_t S, const double* mean,
                                                   const double* var, const double* maxvar, double r1, double r2,
-                                                  double rd, int mode, double* Sm) {
+                                                  double rd, const double* reg_dev, int mode, double* Sm) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)P * P) return;
+    if (reg_dev) {  // r1, r2, r1 delta from device memory (graph replays)
+        r1 = reg_dev[0];
+        r2 = reg_dev[1];
+        rd = reg_dev[2];
+    }
     const int i = (int)(t % P), j = (int)(t / P);
     const double inv = 1.0 / (double)S;
     double re = (g_at(a, 2 * i, 2 * j) + g_at(a, 2 * i + 1, 2 * j + 1)) * inv;
     double im = (g_at(a, 2 * i, 2 * j + 1) - g_at(a, 2 * i + 1, 2 * j)) * inv;
-    const double mri = mean[2 * i], mii = mean[2 * i + 1], mrj = mean[2 * j], mij = mean[2 * j + 1];
-    re -= mri * mrj + mii * mij;  // conj(mu_i) mu_j
-    im -= mri * mij - mii * mrj;
+    if (!a.mean) {  // uncentred products: subtract conj(mu_i) mu_j
+        const double mri = mean[2 * i], mii = mean[2 * i + 1], mrj = mean[2 * j], mij = mean[2 * j + 1];
+        re -= mri * mrj + mii * mij;
+        im -= mri * mij - mii * mrj;
+    }
     if (i == j) {
         const double f = i < n_neural ? r1 : r1 + rd;
         if (mode == 0) re += f * var[i] + r2 * (*maxvar);
@@ -234,11 +246,11 @@ hipError_t launch_syrk<double>(const SyrkArgs<double>& a, hipStream_t st) {
 
 template <typename R>
 hipError_t launch_s_from_g(const SyrkArgs<R>& a, int P, int n_neural, int64_t S, const double* mean,
-                           const double* var, const double* maxvar, double r1, double r2, double rd, int mode,
-                           double* Sm, hipStream_t st) {
+                           const double* var, const double* maxvar, double r1, double r2, double rd,
+                           const double* reg_dev, int mode, double* Sm, hipStream_t st) {
     const int64_t n = (int64_t)P * P;
     hipLaunchKernelGGL(k_s_from_g<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, P, n_neural, S, mean,
-                       var, maxvar, r1, r2, rd, mode, Sm);
+                       var, maxvar, r1, r2, rd, reg_dev, mode, Sm);
     return hipGetLastError();
 }
 
@@ -250,7 +262,8 @@ hipError_t launch_dense_mv(const double* Sm, const double* x, double* y, int P, 
 
 #define DENSE_INST(R)                                                                                           \
     template hipError_t launch_s_from_g<R>(const SyrkArgs<R>&, int, int, int64_t, const double*, const double*, \
-                                           const double*, double, double, double, int, double*, hipStream_t);
+                                           const double*, double, double, double, const double*, int, double*,    \
+                                           hipStream_t);
 DENSE_INST(float)
 DENSE_INST(double)
 

@@ -35,6 +35,7 @@ struct rbmgpu_ctx {
     // ---- sampler
     int n_chains = 1, warmup = 0, step = 1, prec = 64, lanes_req = 0, keep_o = 1;
     bool auto_chains = false;  // the autotuner picks n_chains (config n_chains = 0)
+    int train_dense = -1;      // device SR epoch: 1 CG on the explicit S, 0 matrix-free, -1 undecided
     int chains_cap = 1;        // per-chain buffers hold this many chains
     int64_t n_samples = 0;
     double bond_flips = 0.5;
@@ -1655,7 +1656,7 @@ __global__ void __launch_bounds__(1024) k_epoch_stats(const double* emom, const 
 // with the CG regularisation (cg_solver.cpp:276-281), mode 1 with the
 // direct solver's (direct_solver.cpp:45-53)
 template <typename R>
-int build_S_t(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
+int build_S_t(rbmgpu_ctx* c, double r1, double r2, double rd, int mode, const double* reg_dev) {
     rbmgpu_ctx* ctx = c;
     SyrkArgs<R> a{};
     a.X = (const R*)c->d_O;
@@ -1676,13 +1677,14 @@ int build_S_t(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
     }
     a.part = (R*)c->d_gpart;
     a.wsx = c->weighted ? c->d_wsx : nullptr;
+    a.mean = c->d_omean;
     HIPCK(launch_syrk<R>(a, c->stream));
-    HIPCK(launch_s_from_g<R>(a, (int)c->P, (int)c->P_rbm, c->S, c->d_omean, c->d_ovar, c->d_maxvar, r1, r2, rd, mode,
-                             c->d_Smat, c->stream));
+    HIPCK(launch_s_from_g<R>(a, (int)c->P, (int)c->P_rbm, c->S, c->d_omean, c->d_ovar, c->d_maxvar, r1, r2, rd,
+                             reg_dev, mode, c->d_Smat, c->stream));
     return 0;
 }
 
-int build_S(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
+int build_S(rbmgpu_ctx* c, double r1, double r2, double rd, int mode, const double* reg_dev = nullptr) {
     rbmgpu_ctx* ctx = c;
     if (c->structured || !c->d_O)
         return fail(c, "explicit S needs the O matrix: set keep_o (gpu.keep_o = true) for the cg-single / direct "
@@ -1692,7 +1694,8 @@ int build_S(rbmgpu_ctx* c, double r1, double r2, double rd, int mode) {
         if (int rc = compute_moments(c)) return rc;
     if (!c->d_Smat)
         if (int rc = dalloc(ctx, &c->d_Smat, c->P * c->P * 16)) return rc;
-    return c->prec == RBMGPU_FP32 ? build_S_t<float>(c, r1, r2, rd, mode) : build_S_t<double>(c, r1, r2, rd, mode);
+    return c->prec == RBMGPU_FP32 ? build_S_t<float>(c, r1, r2, rd, mode, reg_dev)
+                                  : build_S_t<double>(c, r1, r2, rd, mode, reg_dev);
 }
 
 // x = A^-1 b for the dense Hermitian A (column-major) on the host: Cholesky
@@ -1892,6 +1895,20 @@ struct DevArgsScope {
 
 // the whole epoch with n_cg CG iterations queued (launches past convergence
 // return at once)
+// one matrix-vector product of the SR epoch's CG: the regularised S.x
+// matrix-free, or the explicit S (train_dense); cg: the CG step follows
+int train_mv(rbmgpu_ctx* c, const double* x, double* y, const CgStatus* st, bool cg) {
+    rbmgpu_ctx* ctx = c;
+    if (c->train_dense != 1) return sx(c, x, y, 0, 0, 0, st, cg ? c->d_pdot : nullptr, cg);
+    HIPCK(launch_dense_mv(c->d_Smat, x, y, (int)c->P, cg ? c->d_pdot : nullptr, st, c->stream));
+    if (cg) {
+        hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, c->d_pdot, (int)((c->P + 3) / 4), c->d_x,
+                           c->d_r, c->d_p, c->d_Ap, (int)c->P, c->d_st);
+        HIPCK(hipGetLastError());
+    }
+    return 0;
+}
+
 int train_body(rbmgpu_ctx* c, int64_t n_cg) {
     rbmgpu_ctx* ctx = c;
     DevArgsScope scope(c);
@@ -1901,12 +1918,18 @@ int train_body(rbmgpu_ctx* c, int64_t n_cg) {
     hipLaunchKernelGGL(k_sr_grad, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->d_emom, c->d_eo,
                        c->d_omean, (int)c->P, c->d_b);
     HIPCK(hipGetLastError());
-    if (int rc = sx(c, c->d_x, c->d_Ap, 0, 0, 0, nullptr)) return rc;
+    if (c->train_dense == 1) {
+        // the explicit S with this epoch's regularisation (DevPar::reg)
+        const double* reg = reinterpret_cast<const double*>(reinterpret_cast<const char*>(c->d_par) +
+                                                            offsetof(DevPar, reg));
+        if (int rc = build_S(c, 0, 0, 0, 0, reg)) return rc;
+    }
+    if (int rc = train_mv(c, c->d_x, c->d_Ap, nullptr, false)) return rc;
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1024), 0, c->stream, c->d_b, c->d_Ap, c->d_r, c->d_p, (int)c->P,
                        c->d_st, (int64_t)0, 0.0, (const DevPar*)c->d_par);
     HIPCK(hipGetLastError());
     for (int64_t i = 0; i < n_cg; i++)
-        if (int rc = sx(c, c->d_p, c->d_Ap, 0, 0, 0, c->d_st, c->d_pdot, true)) return rc;
+        if (int rc = train_mv(c, c->d_p, c->d_Ap, c->d_st, true)) return rc;
     return train_tail(c);
 }
 
@@ -1914,8 +1937,24 @@ int train_body(rbmgpu_ctx* c, int64_t n_cg) {
 int train_more(rbmgpu_ctx* c, int64_t n_cg) {
     DevArgsScope scope(c);
     for (int64_t i = 0; i < n_cg; i++)
-        if (int rc = sx(c, c->d_p, c->d_Ap, 0, 0, 0, c->d_st, c->d_pdot, true)) return rc;
+        if (int rc = train_mv(c, c->d_p, c->d_Ap, c->d_st, true)) return rc;
     return train_tail(c);
+}
+
+// the SR epoch's CG on the explicit S when that is cheaper: one SYRK of O
+// (2 S (2P)^2 flops: MFMA at ~40 TFLOP/s in FP32, VALU at ~15 in FP64) and
+// then per iteration a P x P complex mat-vec (~10 us at these sizes)
+// against a pass over O per iteration (S ld 8 or 16 bytes at ~6 TB/s plus
+// ~15 us of launches); the same trade as the reference's cg-dynamic
+// (cg_solver.cpp:31-46), with this device's rates
+int decide_train_dense(const rbmgpu_ctx* c, int64_t max_iter) {
+    if (c->structured) return 0;  // no O matrix
+    const double S = (double)c->n_samples, P = (double)c->P, ld = (double)c->ld;
+    const double rsz = c->prec == RBMGPU_FP32 ? 8.0 : 16.0;
+    const double syrk = 2.0 * S * (2 * P) * (2 * P) / (c->prec == RBMGPU_FP32 ? 40e12 : 15e12);
+    const double dense = syrk + (double)max_iter * (P * P * 16.0 / 4e12 + 10e-6);
+    const double mfree = (double)max_iter * (S * ld * rsz / 6e12 + 15e-6);
+    return dense < mfree ? 1 : 0;
 }
 
 void drop_graphs(rbmgpu_ctx* c) {
@@ -1994,6 +2033,7 @@ int rbmgpu_create(const rbmgpu_config* cfg, rbmgpu_ctx** out) {
     c->seed = cfg->seed;
     c->prec = cfg->precision;
     c->lanes_req = cfg->lanes_per_chain;
+    c->train_dense = cfg->sr_explicit_s == 0 || cfg->sr_explicit_s == 1 ? cfg->sr_explicit_s : -1;
     c->keep_o = cfg->keep_o;
     if (c->prec != RBMGPU_FP32 && c->prec != RBMGPU_FP64) return bad("precision must be 32 or 64");
     if (c->n_chains < 1 || c->n_samples < 1 || c->step < 1 || c->warmup < 0)
@@ -2300,6 +2340,8 @@ int rbmgpu_train_epoch(rbmgpu_ctx* c, uint32_t epoch, const rbmgpu_sr_params* sp
     // cg_solver.cpp:48-60: max_iter = min(2P, max_iter), 0 -> 2P; rtol 0 -> 1e-16
     int64_t mi = std::min<int64_t>(2 * c->P, sp->max_iter);
     if (mi <= 0) mi = 2 * c->P;
+    if (c->train_dense < 0) c->train_dense = decide_train_dense(c, mi);
+    if (c->structured) c->train_dense = 0;  // no O matrix to build S from
     DevPar* hp = (DevPar*)c->h_par;
     hp->epoch = epoch;
     hp->pad = 0;
@@ -2383,6 +2425,7 @@ int rbmgpu_train_epoch(rbmgpu_ctx* c, uint32_t epoch, const rbmgpu_sr_params* sp
         out->updated = hs->ok;
         out->graph_replayed = sp->graph && !c->graphs.empty();
         out->ms = ms;
+        out->explicit_s = c->train_dense == 1 ? 1 : 0;
     }
     return 0;
 }

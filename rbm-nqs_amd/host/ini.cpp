@@ -147,6 +147,9 @@ const std::map<std::string, setter>& table() {
         {"gpu.lanes_per_chain", [](config& c, const std::string& v) { c.gpu_lanes = std::stoi(v); }},
         {"gpu.keep_o", [](config& c, const std::string& v) { c.gpu_keep_o = boolean(v); }},
         {"gpu.auto_chains", [](config& c, const std::string& v) { c.gpu_auto_chains = boolean(v); }},
+        {"gpu.explicit_s", [](config& c, const std::string& v) {
+             c.gpu_explicit_s = v == "auto" ? -1 : (boolean(v) ? 1 : 0);
+         }},
         {"learn.hidden", [](config& c, const std::string& v) { c.learn_hidden = std::stoi(v); }},
         {"learn.fraction", [](config& c, const std::string& v) { c.learn_fraction = std::stod(v); }},
         {"learn.lr", [](config& c, const std::string& v) { c.learn_lr = std::stod(v); }},

@@ -75,6 +75,7 @@ struct config {
     // materialise the O matrix even where the structured S.x applies
     bool gpu_keep_o = false;
     bool gpu_auto_chains = false;  // the autotuner picks the chain count
+    int gpu_explicit_s = -1;       // SR epoch CG on the explicit S: 1 on, 0 off, -1 cost model
     // learned cluster proposal (new keys; the reference has none)
     int learn_hidden = 0;          // hidden width of the proposal network, 0: off
     double learn_fraction = 0.5;   // probability of a learned move per slot

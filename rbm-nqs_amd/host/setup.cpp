@@ -175,6 +175,7 @@ rbmgpu_config setup::gpu_config() const {
     const bool explicit_S = cfg_.opt_type == "SR" && (m == "cg-single" || m == "direct" || m == "cg-dynamic");
     // exact enumeration weights the samples, which the structured S.x does not do
     c.keep_o = (cfg_.gpu_keep_o || explicit_S || cfg_.sa_type == "full") ? 1 : 0;
+    c.sr_explicit_s = cfg_.gpu_explicit_s;
     c.learn_hidden = cfg_.learn_hidden;
     c.learn_fraction = cfg_.learn_fraction;
     c.cluster_kmax = cluster_kmax_;

@@ -31,7 +31,7 @@ class EpochStats(C.Structure):
     _fields_ = [("e_mean", C.c_double * 2), ("e_var", C.c_double), ("e_var_binned", C.c_double),
                 ("e_tau", C.c_double), ("e_std", C.c_double), ("n_bins", C.c_int64), ("acceptance", C.c_double),
                 ("cg_itn", C.c_int64), ("cg_rs", C.c_double), ("max_dw", C.c_double), ("updated", C.c_int32),
-                ("graph_replayed", C.c_int32), ("ms", C.c_double)]
+                ("graph_replayed", C.c_int32), ("ms", C.c_double), ("explicit_s", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Moments(C.Structure):
@@ -350,7 +350,8 @@ class Gpu:
         self._chk(libs()[0].rbmgpu_train_epoch(self.c, C.c_uint32(epoch), C.byref(p), C.byref(st)))
         return dict(mean=complex(st.e_mean[0], st.e_mean[1]), var=st.e_var, var_binned=st.e_var_binned,
                     tau=st.e_tau, std=st.e_std, n_bins=st.n_bins, acc=st.acceptance, itn=st.cg_itn, rs=st.cg_rs,
-                    max_dw=st.max_dw, updated=bool(st.updated), graph=bool(st.graph_replayed), ms=st.ms)
+                    max_dw=st.max_dw, updated=bool(st.updated), graph=bool(st.graph_replayed), ms=st.ms,
+                    explicit_s=bool(st.explicit_s))
 
     def sr_solve_ex(self, method, b, x0, r1, r2, rd, max_iter=0, rtol=0.0):
         """rbmgpu_sr_solve_ex: method 0 matrix-free CG, 1 CG on the explicit S

@@ -84,7 +84,9 @@ def test_train_epoch_graph_eager_and_host_path(name):
         # the host path starts its CG from the device's own warm start, the
         # previous epoch's step recovered from the parameters, so each epoch
         # is compared on its own (no drift carried over)
-        x, itn, _ = B.sr_solve(F, xA, R1 * 0.9 ** ep, R2, RD, **kw)
+        # the same CG form as the device epoch (matrix-free, or on the
+        # explicit S when the device found it cheaper)
+        x, itn, _ = B.sr_solve_ex(1 if sa["explicit_s"] else 0, F, xA, R1 * 0.9 ** ep, R2, RD, **kw)
         assert itn == 6
         pb1 = pa - LR * x
         err = np.max(np.abs(pb1 - pa1)) / np.abs(LR * x).max()
