@@ -255,13 +255,16 @@ struct SyrkArgs {
     int64_t S, kper;
     int ncol, nvalid, ntile, npairs, ksplit;  // nvalid = 2 P: columns past it read as 0
     R* part;                                  // ksplit x npairs x 64 x 64
+    double* G;                                // npairs x 64 x 64: the slices summed
     const double* wsx;                        // non-null: rows scaled by sqrt(S w_k / sum w)
     const double* mean;                       // non-null: columns centred by <O> (2 P doubles) before the
                                               // products, so S needs no <O><O>^H subtraction
 };
+// the partial tiles, then G (ncol a multiple of 4; a ksplit that is a
+// multiple of 8 keeps each K slice on one XCD)
 template <typename R>
 hipError_t launch_syrk(const SyrkArgs<R>& a, hipStream_t st);
-// S (P x P complex, column-major) from the partials: mode 0 the CG
+// S (P x P complex, column-major) from G: mode 0 the CG
 // regularisation, mode 1 the direct solver's; reg_dev non-null: r1, r2, rd
 // read from device memory instead
 template <typename R>

@@ -8,14 +8,20 @@
 // real SYRK and
 //   Re S_ij ~ G[2i][2j] + G[2i+1][2j+1],  Im S_ij ~ G[2i][2j+1] - G[2i+1][2j].
 // k_syrk_*: one block per upper-triangle 64 x 64 tile of G and K slice
-// (sample range); 32-sample chunks of both panels staged in LDS; FP32 on
-// MFMA (each of the 4 waves owns a 32 x 32 quarter of the tile as 2 x 2
-// v_mfma_f32_16x16x4_f32 tiles), FP64 by VALU FMAs (4 x 4 per thread);
-// each K slice writes its own partial tile (no atomics, a fixed summation
-// order).  k_s_from_g sums the slices in FP64 and forms S.  The columns are
-// centred by <O> while they are staged, so the products form the
-// covariance directly: S = <O^H O> - <O>^H <O> would cancel catastrophically
-// in FP32 where |<O>|^2 >> Var(O) (saturated units of converged states).
+// (sample range).  Blocks are mapped XCD-aware: the dispatcher hands block
+// b to XCD b % 8, and every tile of one K slice runs on the same XCD, so
+// the slice's O rows come from HBM into that XCD's L2 once and the other
+// tiles re-read them there.  FP32 on MFMA: 32-sample chunks of both panels
+// go through registers into a double-buffered LDS stage (the global loads
+// of chunk c + 1 are in flight while chunk c is multiplied); each of the 4
+// waves owns a 32 x 32 quarter of the tile as 2 x 2
+// v_mfma_f32_16x16x4_f32 tiles.  FP64 by VALU FMAs (4 x 4 per thread).
+// Each K slice writes its own partial tile (no atomics, a fixed summation
+// order); k_g_sum adds the slices in FP64 and k_s_from_g forms S.  The
+// columns are centred by <O> while they are staged, so the products form
+// the covariance directly: S = <O^H O> - <O>^H <O> would cancel
+// catastrophically in FP32 where |<O>|^2 >> Var(O) (saturated units of
+// converged states).
 //
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; accumulator r of lane l is D[4(l/16)+r][l%16].
@@ -29,6 +35,7 @@ typedef float v4f_d __attribute__((ext_vector_type(4)));
 
 constexpr int kGT = 64;     // G tile edge
 constexpr int kGChunk = 32; // samples per LDS chunk
+constexpr int kGRow = 80;   // FP32 LDS row stride: the 4 rows one MFMA operand read spans sit 16 banks apart
 
 // tile pair q of the upper triangle (ti <= tj), row-major over ti
 __device__ inline void pair_of(int q, int nt, int& ti, int& tj) {
@@ -41,62 +48,136 @@ __device__ inline void pair_of(int q, int nt, int& ti, int& tj) {
     tj = ti + rem;
 }
 
-// the 64 x 64 tile's panels for samples kc .. kc + 31 into LDS (zeros past
-// the slice end and past the valid columns)
-template <typename R>
-__device__ inline void syrk_stage(const SyrkArgs<R>& a, R (*As)[kGT + 1], R (*Bs)[kGT + 1], int c0, int c1,
-                                  int64_t kc, int64_t k0, int64_t k1) {
+// block -> (tile pair q, K slice): with ksplit a multiple of 8 the K slices
+// are dealt to the 8 XCDs round-robin and all npairs tiles of a slice share
+// one XCD (block b runs on XCD b % 8)
+__device__ inline void syrk_work(int npairs, int ksplit, int& q, int& slice) {
+    const int b = blockIdx.x;
+    if ((ksplit & 7) == 0) {
+        const int x = b & 7, r = b >> 3;
+        slice = (r / npairs) * 8 + x;
+        q = r % npairs;
+    } else {
+        q = b % npairs;
+        slice = b / npairs;
+    }
+}
+
+// FP64: the 64 x 64 tile's panels for samples kc .. kc + 31 into LDS (zeros
+// past the slice end and past the valid columns)
+__device__ inline void syrk_stage_f64(const SyrkArgs<double>& a, double (*As)[kGT + 1], double (*Bs)[kGT + 1],
+                                      int c0, int c1, int64_t kc, int64_t k0, int64_t k1) {
 #pragma unroll
     for (int q = 0; q < kGChunk * kGT / 256; q++) {
         const int e = threadIdx.x + 256 * q, s = e / kGT, c = e % kGT;
         const int64_t k = kc + s;
         const bool in = k < k1;
         const int64_t row = (in ? k : k0) * a.ncol;
-        const R f = a.wsx ? (R)sqrt(a.wsx[in ? k : k0]) : (R)1;
+        const double f = a.wsx ? sqrt(a.wsx[in ? k : k0]) : 1.0;
         const bool va = in && c0 + c < a.nvalid, vb = in && c1 + c < a.nvalid;
-        const R ma = a.mean && va ? (R)a.mean[c0 + c] : (R)0, mb = a.mean && vb ? (R)a.mean[c1 + c] : (R)0;
-        As[s][c] = va ? f * (a.X[row + c0 + c] - ma) : (R)0;
-        Bs[s][c] = vb ? f * (a.X[row + c1 + c] - mb) : (R)0;
+        const double ma = a.mean && va ? a.mean[c0 + c] : 0.0, mb = a.mean && vb ? a.mean[c1 + c] : 0.0;
+        As[s][c] = va ? f * (a.X[row + c0 + c] - ma) : 0.0;
+        Bs[s][c] = vb ? f * (a.X[row + c1 + c] - mb) : 0.0;
+    }
+}
+
+// FP32 staging: thread t carries columns 4 (t % 16) .. +3 of both panels for
+// chunk rows t / 16 and t / 16 + 16
+struct SyrkFetch {
+    float4 a[2], b[2];
+};
+__device__ inline float4 centred4(const float* X, int64_t row, int col, int ncol, int nvalid, const float* m,
+                                  float f) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (col < ncol) v = *reinterpret_cast<const float4*>(X + row + col);
+    v.x = col + 0 < nvalid ? f * (v.x - m[0]) : 0.f;
+    v.y = col + 1 < nvalid ? f * (v.y - m[1]) : 0.f;
+    v.z = col + 2 < nvalid ? f * (v.z - m[2]) : 0.f;
+    v.w = col + 3 < nvalid ? f * (v.w - m[3]) : 0.f;
+    return v;
+}
+__device__ inline void syrk_fetch(const SyrkArgs<float>& a, SyrkFetch& r, int ca, int cb, bool diag,
+                                  const float* ma, const float* mb, int64_t kc, int64_t k1) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int64_t k = kc + (threadIdx.x >> 4) + 16 * h;
+        if (k < k1) {
+            const int64_t row = k * a.ncol;
+            const float f = a.wsx ? (float)sqrt(a.wsx[k]) : 1.f;
+            r.a[h] = centred4(a.X, row, ca, a.ncol, a.nvalid, ma, f);
+            r.b[h] = diag ? r.a[h] : centred4(a.X, row, cb, a.ncol, a.nvalid, mb, f);
+        } else {
+            r.a[h] = r.b[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
 }
 
 // FP32: MFMA, each wave a 32 x 32 quarter of the tile (2 x 2 MFMA tiles)
 __global__ void __launch_bounds__(256) k_syrk_f32(SyrkArgs<float> a) {
-    __shared__ float As[kGChunk][kGT + 1];
-    __shared__ float Bs[kGChunk][kGT + 1];
+    __shared__ __align__(16) float As[2][kGChunk][kGRow];
+    __shared__ __align__(16) float Bs[2][kGChunk][kGRow];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
     const int wi = w >> 1, wj = w & 1;
-    int ti, tj;
-    pair_of(blockIdx.x, a.ntile, ti, tj);
-    const int64_t k0 = (int64_t)blockIdx.y * a.kper, k1 = k0 + a.kper < a.S ? k0 + a.kper : a.S;
+    int q, slice, ti, tj;
+    syrk_work(a.npairs, a.ksplit, q, slice);
+    pair_of(q, a.ntile, ti, tj);
+    const bool diag = ti == tj;
+    const int64_t k0 = (int64_t)slice * a.kper, k1 = k0 + a.kper < a.S ? k0 + a.kper : a.S;
+    const int cq = 4 * (tid & 15), ca = ti * kGT + cq, cb = tj * kGT + cq;
+    float ma[4], mb[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        ma[i] = a.mean && ca + i < a.nvalid ? (float)a.mean[ca + i] : 0.f;
+        mb[i] = a.mean && cb + i < a.nvalid ? (float)a.mean[cb + i] : 0.f;
+    }
     v4f_d acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
         for (int y = 0; y < 2; y++) acc[x][y] = v4f_d{0, 0, 0, 0};
+    SyrkFetch r;
+    if (k0 < k1) {
+        syrk_fetch(a, r, ca, cb, diag, ma, mb, k0, k1);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            *reinterpret_cast<float4*>(&As[0][(tid >> 4) + 16 * h][cq]) = r.a[h];
+            *reinterpret_cast<float4*>(&Bs[0][(tid >> 4) + 16 * h][cq]) = r.b[h];
+        }
+    }
+    __syncthreads();
+    int buf = 0;
     for (int64_t kc = k0; kc < k1; kc += kGChunk) {
-        syrk_stage<float>(a, As, Bs, ti * kGT, tj * kGT, kc, k0, k1);
-        __syncthreads();
+        const bool more = kc + kGChunk < k1;
+        if (more) syrk_fetch(a, r, ca, cb, diag, ma, mb, kc + kGChunk, k1);
 #pragma unroll
         for (int kk = 0; kk < kGChunk; kk += 4) {
-            const float a0 = As[kk + lq][wi * 32 + lr], a1 = As[kk + lq][wi * 32 + 16 + lr];
-            const float b0 = Bs[kk + lq][wj * 32 + lr], b1 = Bs[kk + lq][wj * 32 + 16 + lr];
+            const float a0 = As[buf][kk + lq][wi * 32 + lr], a1 = As[buf][kk + lq][wi * 32 + 16 + lr];
+            const float b0 = Bs[buf][kk + lq][wj * 32 + lr], b1 = Bs[buf][kk + lq][wj * 32 + 16 + lr];
             acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
             acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
             acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
         }
+        if (more) {
+            // the other buffer was last read before the previous barrier
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                *reinterpret_cast<float4*>(&As[buf ^ 1][(tid >> 4) + 16 * h][cq]) = r.a[h];
+                *reinterpret_cast<float4*>(&Bs[buf ^ 1][(tid >> 4) + 16 * h][cq]) = r.b[h];
+            }
+        }
         __syncthreads();
+        buf ^= 1;
     }
-    float* out = a.part + ((int64_t)blockIdx.y * a.npairs + blockIdx.x) * (kGT * kGT);
+    float* out = a.part + ((int64_t)slice * a.npairs + q) * (kGT * kGT);
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
         for (int y = 0; y < 2; y++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = wi * 32 + x * 16 + 4 * lq + r, j = wj * 32 + y * 16 + lr;
-                out[i * kGT + j] = acc[x][y][r];
+            for (int rr = 0; rr < 4; rr++) {
+                const int i = wi * 32 + x * 16 + 4 * lq + rr, j = wj * 32 + y * 16 + lr;
+                out[i * kGT + j] = acc[x][y][rr];
             }
 }
 
@@ -106,16 +187,17 @@ __global__ void __launch_bounds__(256) k_syrk_f64(SyrkArgs<double> a) {
     __shared__ double As[kGChunk][kGT + 1];
     __shared__ double Bs[kGChunk][kGT + 1];
     const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-    int ti, tj;
-    pair_of(blockIdx.x, a.ntile, ti, tj);
-    const int64_t k0 = (int64_t)blockIdx.y * a.kper, k1 = k0 + a.kper < a.S ? k0 + a.kper : a.S;
+    int q, slice, ti, tj;
+    syrk_work(a.npairs, a.ksplit, q, slice);
+    pair_of(q, a.ntile, ti, tj);
+    const int64_t k0 = (int64_t)slice * a.kper, k1 = k0 + a.kper < a.S ? k0 + a.kper : a.S;
     double acc[4][4];
 #pragma unroll
     for (int x = 0; x < 4; x++)
 #pragma unroll
         for (int y = 0; y < 4; y++) acc[x][y] = 0;
     for (int64_t kc = k0; kc < k1; kc += kGChunk) {
-        syrk_stage<double>(a, As, Bs, ti * kGT, tj * kGT, kc, k0, k1);
+        syrk_stage_f64(a, As, Bs, ti * kGT, tj * kGT, kc, k0, k1);
         __syncthreads();
 #pragma unroll 4
         for (int k = 0; k < kGChunk; k++) {
@@ -131,14 +213,25 @@ __global__ void __launch_bounds__(256) k_syrk_f64(SyrkArgs<double> a) {
         }
         __syncthreads();
     }
-    double* out = a.part + ((int64_t)blockIdx.y * a.npairs + blockIdx.x) * (kGT * kGT);
+    double* out = a.part + ((int64_t)slice * a.npairs + q) * (kGT * kGT);
 #pragma unroll
     for (int x = 0; x < 4; x++)
 #pragma unroll
         for (int y = 0; y < 4; y++) out[(4 * ty + x) * kGT + tx + 16 * y] = acc[x][y];
 }
 
-// G[u][v] summed over the K slices in FP64 (u, v < 2 ld; symmetric)
+// G = the K slices' partial tiles summed in FP64, in slice order
+template <typename R>
+__global__ void __launch_bounds__(256) k_g_sum(SyrkArgs<R> a) {
+    const int64_t n = (int64_t)a.npairs * kGT * kGT;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int k = 0; k < a.ksplit; k++) s += (double)a.part[(int64_t)k * n + e];
+        a.G[e] = s;
+    }
+}
+
+// G[u][v] (u, v < 2 ld; symmetric) from the summed tiles
 template <typename R>
 __device__ inline double g_at(const SyrkArgs<R>& a, int u, int v) {
     int tu = u / kGT, tv = v / kGT;
@@ -151,10 +244,7 @@ __device__ inline double g_at(const SyrkArgs<R>& a, int u, int v) {
         tv = tt;
     }
     const int q = tu * a.ntile - tu * (tu - 1) / 2 + (tv - tu);
-    const R* p = a.part + (int64_t)q * (kGT * kGT) + (u % kGT) * kGT + (v % kGT);
-    double s = 0;
-    for (int k = 0; k < a.ksplit; k++) s += (double)p[(int64_t)k * a.npairs * (kGT * kGT)];
-    return s;
+    return a.G[(int64_t)q * (kGT * kGT) + (u % kGT) * kGT + (v % kGT)];
 }
 
 // S from G: thread (i, j) of the P x P complex matrix (column-major),
@@ -231,17 +321,25 @@ __global__ void __launch_bounds__(256) k_dense_mv(const double* Sm, const double
     }
 }
 
-template <>
-hipError_t launch_syrk<float>(const SyrkArgs<float>& a, hipStream_t st) {
-    if (a.ntile < 1 || a.ksplit < 1 || a.kper % kGChunk) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_syrk_f32, dim3((unsigned)a.npairs, (unsigned)a.ksplit), dim3(256), 0, st, a);
+template <typename R>
+static hipError_t g_sum(const SyrkArgs<R>& a, hipStream_t st) {
+    const int64_t n = (int64_t)a.npairs * kGT * kGT;
+    hipLaunchKernelGGL(k_g_sum<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 template <>
+hipError_t launch_syrk<float>(const SyrkArgs<float>& a, hipStream_t st) {
+    if (a.ntile < 1 || a.ksplit < 1 || a.kper % kGChunk || a.ncol % 4 || !a.G) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_syrk_f32, dim3((unsigned)(a.npairs * a.ksplit)), dim3(256), 0, st, a);
+    if (hipError_t e = hipGetLastError()) return e;
+    return g_sum(a, st);
+}
+template <>
 hipError_t launch_syrk<double>(const SyrkArgs<double>& a, hipStream_t st) {
-    if (a.ntile < 1 || a.ksplit < 1 || a.kper % kGChunk) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)a.npairs, (unsigned)a.ksplit), dim3(256), 0, st, a);
-    return hipGetLastError();
+    if (a.ntile < 1 || a.ksplit < 1 || a.kper % kGChunk || !a.G) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)(a.npairs * a.ksplit)), dim3(256), 0, st, a);
+    if (hipError_t e = hipGetLastError()) return e;
+    return g_sum(a, st);
 }
 
 template <typename R>

@@ -825,41 +825,133 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs<R> a) {
 // O column of one sample (rbm_base.cpp:194-217, rbm_symmetry.cpp:97-130):
 // [ sum_s sigma_{perm_s(basis_i)} | sum_s tanh theta(a,s) |
 //   sum_s (P_s sigma)_j tanh theta(a,s) at j + a*N ]
+// A block takes batches of kOBatch samples (their tanh rows and states in
+// LDS, staged by unrolled register loads; the permutation tables staged
+// once per block as bytes, N <= 128); 256 threads loop over the (sample,
+// site) items of a batch.  Weight entries:
+// item (u, j) runs over the symmetries once per 4 alpha channels, the spin
+// (P_s sigma)_j read as a bit of the state words in registers, the 4
+// channels' tanh values read unconditionally (no per-channel branch between
+// the LDS reads), so the stores of a wave cover consecutive j.
+constexpr int kOBatch = 8;
+template <typename R>
+__host__ __device__ inline size_t build_O_lds(const Dims& d) {
+    return (size_t)kOBatch * d.H * sizeof(cplx<R>) + (size_t)kOBatch * d.nw * sizeof(uint64_t) +
+           (size_t)d.n_symm * (d.N + d.n_vb);
+}
+
+template <typename R>
+__device__ inline R spin_of(uint64_t w0, uint64_t w1, int f) {
+    return (((f < 64 ? w0 : w1) >> (f & 63)) & 1) ? (R)-1 : (R)1;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_build_O(OArgs<R> a) {
     extern __shared__ __align__(16) char smem[];
     const Dims d = a.d;
-    float* sg = reinterpret_cast<float*>(smem);
-    cplx<R>* tr = reinterpret_cast<cplx<R>*>(sg + 128);
-    int64_t k = blockIdx.x;
-    if (k >= a.S) return;
-    for (int f = threadIdx.x; f < d.N; f += blockDim.x)
-        sg[f] = ((a.states[k * d.nw + (f >> 6)] >> (f & 63)) & 1) ? -1.f : 1.f;
-    for (int h = threadIdx.x; h < d.H; h += blockDim.x) tr[h] = a.T[k * d.H + h];
-    __syncthreads();
-    cplx<R>* o = a.O + k * a.ld;
-    int nvb = d.n_vb, na = d.n_alpha;
-    for (int p = a.P + threadIdx.x; p < a.ld; p += blockDim.x) o[p] = {0, 0};
-    if (a.Opf)  // Pfaffian block (pfaffian.cpp:399-415), written by the sampler
-        for (int p = a.P_rbm + threadIdx.x; p < a.P; p += blockDim.x) o[p] = a.Opf[k * (a.P - a.P_rbm) + p - a.P_rbm];
-    for (int p = threadIdx.x; p < a.P_rbm; p += blockDim.x) {
-        cplx<R> v{0, 0};
-        if (p < nvb) {
-            R acc = 0;
-            for (int s = 0; s < d.n_symm; s++) acc += (R)sg[a.vb_site[s * nvb + p]];
-            v = {acc, 0};
-        } else if (p < nvb + na) {
-            int al = p - nvb;
-            for (int s = 0; s < d.n_symm; s++) v = v + tr[al + s * na];
-        } else {
-            int q = p - nvb - na, j = q % d.N, al = q / d.N;
-            if (d.n_symm == 1) {
-                v = (R)sg[j] * tr[al];
-            } else {
-                for (int s = 0; s < d.n_symm; s++) v = v + (R)sg[a.invperm[s * d.N + j]] * tr[al + s * na];
+    const int N = d.N, ns = d.n_symm, nvb = d.n_vb, na = d.n_alpha, H = d.H, nw = d.nw;
+    const int nt = blockDim.x;
+    cplx<R>* tr = reinterpret_cast<cplx<R>*>(smem);                     // kOBatch x H
+    uint64_t* stw = reinterpret_cast<uint64_t*>(tr + kOBatch * H);      // kOBatch x nw
+    uint8_t* ip = reinterpret_cast<uint8_t*>(stw + kOBatch * nw);       // ns x N: perm_s^-1(j)
+    uint8_t* vb = ip + ns * N;                                          // ns x nvb: perm_s(basis_i)
+    {
+        constexpr int U = 4;  // loads in flight per thread
+        const int ni = ns * N, nv = ns * nvb;
+        for (int e0 = threadIdx.x; e0 < ni + nv; e0 += U * nt) {
+            int32_t v[U];
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                const int e = e0 + i * nt;
+                v[i] = e < ni ? a.invperm[e] : e < ni + nv ? a.vb_site[e - ni] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < U; i++)
+                if (e0 + i * nt < ni + nv) ip[e0 + i * nt] = (uint8_t)v[i];  // vb follows ip
+        }
+    }
+    const int nb = nvb + na, tail = a.ld - a.P_rbm;
+    const int64_t nbatch = (a.S + kOBatch - 1) / kOBatch;
+    for (int64_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+        const int64_t kb = bt * kOBatch;
+        const int nsb = a.S - kb < kOBatch ? (int)(a.S - kb) : kOBatch;
+        __syncthreads();  // the previous batch's readers are done
+        {
+            constexpr int U = 4;
+            const int nel = nsb * H;
+            const cplx<R>* src = a.T + kb * H;
+            int e0 = threadIdx.x;
+            for (; e0 + (U - 1) * nt < nel; e0 += U * nt) {  // whole groups: U loads in flight
+                cplx<R> v[U];
+#pragma unroll
+                for (int i = 0; i < U; i++) v[i] = src[e0 + i * nt];
+#pragma unroll
+                for (int i = 0; i < U; i++) tr[e0 + i * nt] = v[i];
+            }
+            for (; e0 < nel; e0 += nt) tr[e0] = src[e0];
+            for (int e = threadIdx.x; e < nsb * nw; e += nt) stw[e] = a.states[kb * nw + e];
+        }
+        __syncthreads();
+        for (int it = threadIdx.x; it < nsb * N; it += nt) {
+            const int u = it / N, j = it - u * N;
+            const uint64_t w0 = stw[u * nw], w1 = nw > 1 ? stw[u * nw + 1] : 0;
+            const cplx<R>* tu = tr + u * H;
+            cplx<R>* o = a.O + (kb + u) * a.ld + nb + j;
+            int al0 = 0;
+            for (; al0 + 4 <= na; al0 += 4) {
+                cplx<R> acc[4] = {};
+                int s = 0;
+                for (; s + 4 <= ns; s += 4) {  // the LDS reads of 4 symmetries issued together
+                    int f[4];
+                    cplx<R> t[4][4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) f[i] = ip[(s + i) * N + j];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) t[i][q] = tu[(s + i) * na + al0 + q];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const R sg = spin_of<R>(w0, w1, f[i]);
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc[q] = acc[q] + sg * t[i][q];
+                    }
+                }
+                for (; s < ns; s++) {
+                    const R sg = spin_of<R>(w0, w1, ip[s * N + j]);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[q] = acc[q] + sg * tu[s * na + al0 + q];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) o[(int64_t)(al0 + q) * N] = acc[q];
+            }
+            for (; al0 < na; al0++) {  // channels past the last group of 4
+                cplx<R> acc{0, 0};
+                for (int s = 0; s < ns; s++) acc = acc + spin_of<R>(w0, w1, ip[s * N + j]) * tu[s * na + al0];
+                o[(int64_t)al0 * N] = acc;
             }
         }
-        o[p] = v;
+        // biases, the Pfaffian block (pfaffian.cpp:399-415, written by the
+        // sampler) and the row padding
+        for (int it = threadIdx.x; it < nsb * (nb + tail); it += nt) {
+            const int uu = it / (nb + tail), p = it - uu * (nb + tail);
+            const uint64_t w0 = stw[uu * nw], w1 = nw > 1 ? stw[uu * nw + 1] : 0;
+            const cplx<R>* tu = tr + uu * H;
+            const int64_t k = kb + uu;
+            cplx<R> v{0, 0};
+            int col = p;
+            if (p < nvb) {
+                R acc = 0;
+                for (int s = 0; s < ns; s++) acc += spin_of<R>(w0, w1, vb[s * nvb + p]);
+                v = {acc, 0};
+            } else if (p < nb) {
+                for (int s = 0; s < ns; s++) v = v + tu[p - nvb + s * na];
+            } else {
+                col = a.P_rbm + (p - nb);
+                if (col < a.P && a.Opf) v = a.Opf[k * (a.P - a.P_rbm) + col - a.P_rbm];
+            }
+            a.O[k * a.ld + col] = v;
+        }
     }
 }
 
@@ -868,42 +960,79 @@ __global__ void __launch_bounds__(256) k_build_O(OArgs<R> a) {
 // sum E conj(O) (accumulated in double)
 template <typename R>
 __global__ void __launch_bounds__(256) k_o_partial(MomArgs<R> a) {
-    // per block of samples: sums of O - K, |O - K|^2 with the shift K = the
-    // block's first O (shifted-data variance: no cancellation when O is
-    // nearly constant, e.g. saturated hidden units) and of E conj(O), each
-    // sample weighted by w (1 without weights).  part row per (block, p):
-    // [K re, K im, sum w d re, sum w d im, sum w |d|^2, sum w E conj(O) re,
-    // im, sum w, sum w^2]; k_o_final merges the blocks pairwise.
-    int64_t k0 = (int64_t)blockIdx.x * a.chunk, k1 = k0 + a.chunk;
-    if (k1 > a.S) k1 = a.S;
-    for (int p = threadIdx.x; p < a.P; p += blockDim.x) {
-        const cplx<R> o0 = a.O[k0 * a.ld + p];
-        const double kr = o0.re, ki = o0.im;
-        double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0, sw = 0, sw2 = 0;
-        for (int64_t k = k0; k < k1; k++) {
-            cplx<R> o = a.O[k * a.ld + p];
-            double ore = o.re, oim = o.im;
-            double ere = a.obs[k * 2 * a.n_groups], eim = a.obs[k * 2 * a.n_groups + 1];
-            const double w = a.w ? a.w[k] : 1.0;
-            const double dr = ore - kr, di = oim - ki;
-            s0 += w * dr;
-            s1 += w * di;
-            s2 += w * (dr * dr + di * di);
-            e0 += w * (ere * ore + eim * oim);  // E * conj(O)
-            e1 += w * (eim * ore - ere * oim);
-            sw += w;
-            sw2 += w * w;
+    // block (chunk b = blockIdx.x, parameters 64 blockIdx.y + lane): sums of
+    // O - K, |O - K|^2 with the shift K = the chunk's first O (shifted-data
+    // variance: no cancellation when O is nearly constant, e.g. saturated
+    // hidden units) and of E conj(O), each sample weighted by w (1 without
+    // weights).  Wave w takes samples k0 + w, k0 + w + 4, ...; the 4 waves
+    // add up in LDS.  part row per (b, p): [K re, K im, sum w d re,
+    // sum w d im, sum w |d|^2, sum w E conj(O) re, im, sum w, sum w^2];
+    // k_o_final merges the chunks.
+    __shared__ double red[3][7][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = (int)blockIdx.y * 64 + lane;
+    const bool on = p < a.P;
+    const int64_t k0 = (int64_t)blockIdx.x * a.chunk, k1 = k0 + a.chunk < a.S ? k0 + a.chunk : a.S;
+    const int pp = on ? p : 0;
+    const cplx<R> o0 = a.O[k0 * a.ld + pp];
+    const double kr = o0.re, ki = o0.im;
+    double s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0, sw = 0, sw2 = 0;
+    const int64_t eo = 2 * (int64_t)a.n_groups;
+    auto add = [&](int64_t k, cplx<R> o) {
+        const double ore = o.re, oim = o.im;
+        const double ere = a.obs[k * eo], eim = a.obs[k * eo + 1];
+        const double w = a.w ? a.w[k] : 1.0;
+        const double dr = ore - kr, di = oim - ki;
+        s0 += w * dr;
+        s1 += w * di;
+        s2 += w * (dr * dr + di * di);
+        e0 += w * (ere * ore + eim * oim);  // E * conj(O)
+        e1 += w * (eim * ore - ere * oim);
+        sw += w;
+        sw2 += w * w;
+    };
+    int64_t k = k0 + wv;
+    for (; k + 12 < k1; k += 16) {  // four rows in flight per wave
+        const cplx<R> o_0 = a.O[k * a.ld + pp], o_1 = a.O[(k + 4) * a.ld + pp];
+        const cplx<R> o_2 = a.O[(k + 8) * a.ld + pp], o_3 = a.O[(k + 12) * a.ld + pp];
+        add(k, o_0);
+        add(k + 4, o_1);
+        add(k + 8, o_2);
+        add(k + 12, o_3);
+    }
+    for (; k < k1; k += 4) add(k, a.O[k * a.ld + pp]);
+    if (wv > 0) {
+        double* r = &red[wv - 1][0][lane];
+        r[0] = s0;
+        r[64] = s1;
+        r[128] = s2;
+        r[192] = e0;
+        r[256] = e1;
+        r[320] = sw;
+        r[384] = sw2;
+    }
+    __syncthreads();
+    if (wv == 0 && on) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {  // fixed order: waves 1, 2, 3
+            s0 += red[q][0][lane];
+            s1 += red[q][1][lane];
+            s2 += red[q][2][lane];
+            e0 += red[q][3][lane];
+            e1 += red[q][4][lane];
+            sw += red[q][5][lane];
+            sw2 += red[q][6][lane];
         }
-        double* q = a.part + ((int64_t)blockIdx.x * a.P + p) * kOPartRow;
-        q[0] = kr;
-        q[1] = ki;
-        q[2] = s0;
-        q[3] = s1;
-        q[4] = s2;
-        q[5] = e0;
-        q[6] = e1;
-        q[7] = sw;
-        q[8] = sw2;
+        double* o = a.part + ((int64_t)blockIdx.x * a.P + p) * kOPartRow;
+        o[0] = kr;
+        o[1] = ki;
+        o[2] = s0;
+        o[3] = s1;
+        o[4] = s2;
+        o[5] = e0;
+        o[6] = e1;
+        o[7] = sw;
+        o[8] = sw2;
     }
 }
 

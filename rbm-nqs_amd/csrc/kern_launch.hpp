@@ -75,13 +75,23 @@ hipError_t launch_eval(const EvalArgs<R>& a, Shape s, int block, hipStream_t st)
 
 template <typename R>
 hipError_t launch_build_O(const OArgs<R>& a, hipStream_t st) {
-    size_t lds = 128 * sizeof(float) + (size_t)a.d.H * sizeof(cplx<R>);
-    hipLaunchKernelGGL((k_build_O<R>), dim3((unsigned)a.S), dim3(256), lds, st, a);
+    if (a.S < 1) return hipSuccess;
+    if (a.d.N > 128 || a.d.nw > 2) return hipErrorInvalidValue;
+    const size_t lds = build_O_lds<R>(a.d);
+    if (lds > 65536) {
+        hipError_t e =
+            hipFuncSetAttribute((const void*)k_build_O<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int64_t nbatch = (a.S + kOBatch - 1) / kOBatch;
+    const unsigned blocks = (unsigned)(nbatch < 2048 ? nbatch : 2048);
+    hipLaunchKernelGGL((k_build_O<R>), dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_o_partial(const MomArgs<R>& a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_o_partial<R>), dim3(blocks), dim3(256), 0, st, a);
+    if (blocks < 1 || a.P < 1 || a.chunk < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_o_partial<R>), dim3(blocks, (unsigned)((a.P + 63) / 64)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -733,49 +733,6 @@ float elapsed(rbmgpu_ctx* c, int a, int b) {
 }
 
 // ---- reductions in FP64 (aggregator.cpp:84-239, one rank, unit weights)
-__global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb, int64_t chunk, int64_t S, int P,
-                                                 double* mean, double* var, double* eo,
-                                                 unsigned long long* maxvar) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    // block b (weight W_b = sum w, n_b samples without weights): mean_b = K_b
-    // + sum w d / W_b, M2_b = sum w |d|^2 - |sum w d|^2 / W_b; merged
-    // pairwise, M2 = sum_b M2_b + W_b |mean_b - mean|^2 (Chan, Golub and
-    // LeVeque; the reference accumulates weighted Welford updates,
-    // aggregator.cpp:190-231), divided by W - sum w^2 / W (aggregator.cpp:141-154)
-    double sr = 0, si = 0, er = 0, ei = 0, W = 0, W2 = 0;
-    for (int64_t b = 0; b < nb; b++) {
-        const double* q = part + (b * P + p) * kOPartRow;
-        sr += q[7] * q[0] + q[2];
-        si += q[7] * q[1] + q[3];
-        er += q[5];
-        ei += q[6];
-        W += q[7];
-        W2 += q[8];
-    }
-    const double inv = 1.0 / W;
-    const double mr = sr * inv, mi = si * inv;
-    double m2 = 0;
-    for (int64_t b = 0; b < nb; b++) {
-        const double* q = part + (b * P + p) * kOPartRow;
-        const double n = q[7];
-        if (n == 0) continue;
-        const double dr = q[2] / n, di = q[3] / n;                 // mean_b - K_b
-        const double ur = q[0] + dr - mr, ui = q[1] + di - mi;     // mean_b - mean
-        m2 += (q[4] - n * (dr * dr + di * di)) + n * (ur * ur + ui * ui);
-    }
-    (void)chunk;
-    (void)S;
-    mean[2 * p] = mr;
-    mean[2 * p + 1] = mi;
-    const double den = W - W2 / W;
-    const double v = den > 0 ? fmax(m2 / den, 0.0) : 0.0;
-    var[p] = v;
-    eo[2 * p] = er * inv;
-    eo[2 * p + 1] = ei * inv;
-    atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
-}
-
 // ---- CG on the device, FP64 vectors (cg_solver.cpp:119-189)
 // block-wide sums of x and y, returned to every thread (blockDim a multiple
 // of 64, at most 1024): wave shuffles, then one LDS pass over the waves
@@ -795,6 +752,53 @@ __device__ void block_sum2(double& x, double& y) {
         y += rb[q];
     }
     __syncthreads();  // ra / rb may be written again by the next call
+}
+
+// <O>, Var(O), <E O*> per parameter from k_o_partial's chunks: block p,
+// its threads over the chunks, block sums in a fixed order.  Chunk b
+// (weight W_b = sum w, n_b samples without weights): mean_b = K_b +
+// sum w d / W_b, M2_b = sum w |d|^2 - |sum w d|^2 / W_b; merged as M2 =
+// sum_b M2_b + W_b |mean_b - mean|^2 (Chan, Golub and LeVeque; the
+// reference accumulates weighted Welford updates, aggregator.cpp:190-231),
+// divided by W - sum w^2 / W (aggregator.cpp:141-154)
+__global__ void __launch_bounds__(256) k_o_final(const double* part, int64_t nb, int P, double* mean, double* var,
+                                                 double* eo, unsigned long long* maxvar) {
+    const int p = blockIdx.x;
+    double sr = 0, si = 0, er = 0, ei = 0, W = 0, W2 = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double* q = part + (b * P + p) * kOPartRow;
+        sr += q[7] * q[0] + q[2];
+        si += q[7] * q[1] + q[3];
+        er += q[5];
+        ei += q[6];
+        W += q[7];
+        W2 += q[8];
+    }
+    block_sum2(sr, si);
+    block_sum2(er, ei);
+    block_sum2(W, W2);
+    const double inv = 1.0 / W;
+    const double mr = sr * inv, mi = si * inv;
+    double m2 = 0, dummy = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double* q = part + (b * P + p) * kOPartRow;
+        const double n = q[7];
+        if (n == 0) continue;
+        const double dr = q[2] / n, di = q[3] / n;              // mean_b - K_b
+        const double ur = q[0] + dr - mr, ui = q[1] + di - mi;  // mean_b - mean
+        m2 += (q[4] - n * (dr * dr + di * di)) + n * (ur * ur + ui * ui);
+    }
+    block_sum2(m2, dummy);
+    if (threadIdx.x == 0) {
+        mean[2 * p] = mr;
+        mean[2 * p + 1] = mi;
+        const double den = W - W2 / W;
+        const double v = den > 0 ? fmax(m2 / den, 0.0) : 0.0;
+        var[p] = v;
+        eo[2 * p] = er * inv;
+        eo[2 * p + 1] = ei * inv;
+        atomicMax(maxvar, (unsigned long long)__double_as_longlong(v));
+    }
 }
 
 // Energy moments (aggregator.cpp:84-154 with binning; weights w of an
@@ -1461,9 +1465,8 @@ int compute_moments(rbmgpu_ctx* c) {
             HIPCK(launch_o_partial<double>(a, (int)blocks, c->stream));
         }
         if (int rc = zero_async(c, c->d_maxvar, sizeof(double))) return rc;
-        hipLaunchKernelGGL(k_o_final, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->d_part,
-                           blocks, chunk, c->S, (int)c->P, c->d_omean, c->d_ovar, c->d_eo,
-                           (unsigned long long*)c->d_maxvar);
+        hipLaunchKernelGGL(k_o_final, dim3((unsigned)c->P), dim3(256), 0, c->stream, c->d_part, blocks, (int)c->P,
+                           c->d_omean, c->d_ovar, c->d_eo, (unsigned long long*)c->d_maxvar);
         HIPCK(hipGetLastError());
     }
     if (int rc = record(c, 5)) return rc;
@@ -1665,17 +1668,24 @@ int build_S_t(rbmgpu_ctx* c, double r1, double r2, double rd, int mode, const do
     a.nvalid = (int)(2 * c->P);
     a.ntile = (int)((2 * c->P + 63) / 64);
     a.npairs = a.ntile * (a.ntile + 1) / 2;
-    // about 4 blocks per CU over tiles x sample slices of >= 256 samples
-    int64_t ks = std::max<int64_t>(1, (4 * (int64_t)c->n_cu + a.npairs - 1) / a.npairs);
+    // about 7 blocks per CU over tiles x sample slices of >= 256 samples
+    // (4 stay resident per CU; dynamic dispatch then evens out the CUs),
+    // the slice count a multiple of 8 where the samples allow, so every
+    // slice stays on one XCD
+    int64_t ks = std::max<int64_t>(1, (7 * (int64_t)c->n_cu + a.npairs - 1) / a.npairs);
     ks = std::min<int64_t>(ks, std::max<int64_t>(1, c->S / 256));
+    if (ks >= 8) ks = ks / 8 * 8;
     a.kper = ((c->S + ks - 1) / ks + 31) / 32 * 32;
     a.ksplit = (int)((c->S + a.kper - 1) / a.kper);
-    const int64_t need = (int64_t)a.ksplit * a.npairs * 64 * 64 * (int64_t)sizeof(R);
+    if (a.ksplit % 8 && ks % 8 == 0) a.ksplit = (int)ks;  // trailing slices may be empty
+    const int64_t gbytes = (int64_t)a.npairs * 64 * 64 * (int64_t)sizeof(double);
+    const int64_t need = gbytes + (int64_t)a.ksplit * a.npairs * 64 * 64 * (int64_t)sizeof(R);
     if (need > c->gpart_cap) {
         if (int rc = dalloc(ctx, &c->d_gpart, need)) return rc;
         c->gpart_cap = need;
     }
-    a.part = (R*)c->d_gpart;
+    a.G = (double*)c->d_gpart;
+    a.part = (R*)((char*)c->d_gpart + gbytes);
     a.wsx = c->weighted ? c->d_wsx : nullptr;
     a.mean = c->d_omean;
     HIPCK(launch_syrk<R>(a, c->stream));

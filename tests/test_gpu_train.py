@@ -13,7 +13,8 @@ Per epoch:
     same parameters), so its energy equals the device epoch's to 1e-10, and
     after 6 CG iterations its update w - lr x agrees with the device's to
     1e-9 of max |lr x| in FP64 (F and the warm start differ only by
-    rounding between the two paths), 1e-4 in FP32.
+    rounding between the two paths), in FP32 to 1e-8 in the first epoch
+    and 1e-4 after it.
 """
 import numpy as np
 import pytest
@@ -52,7 +53,8 @@ def _lanes(spec, prec, p):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_train_epoch_graph_eager_and_host_path(name):
-    """6 CG iterations per epoch: graph == eager bitwise, host path 1e-9"""
+    """6 CG iterations per epoch: graph == eager bitwise, host path 1e-9
+    (FP64)"""
     spec, prec = CASES[name]
     print(f"[gpu-test] train_epoch {name}", flush=True)
     rng = np.random.default_rng(4)
@@ -92,12 +94,15 @@ def test_train_epoch_graph_eager_and_host_path(name):
         err = np.max(np.abs(pb1 - pa1)) / np.abs(LR * x).max()
         xA = pa - pa1  # the warm start of the next solve is this epoch's step lr x
         print(f"[gpu-test] epoch {ep}: E/N {sa['mean'].real / m.N:.6f}, update error {err:.1e}", flush=True)
-        # FP64 and the first epoch (zero warm start on both paths): 1e-9.
-        # FP32 later epochs: the recovered warm start differs from the
-        # device's by rounding (~1e-12 relative); the FP32 S.x re-rounds such
-        # inputs to whole FP32 ulps now and then, and 6 CG steps of the
-        # ill-conditioned symmetric system amplify that: stated bound 1e-4
-        assert err <= (1e-9 if prec == 64 or ep == 0 else 1e-4), (ep, err)
+        # FP64: 1e-9.  FP32, first epoch (zero warm start on both paths):
+        # 1e-8; the FP32 S.x rounds its input vector to FP32, so ulp-level
+        # differences between the paths flip roundings: on kitaev6_sym_fp32
+        # a 1e-16 relative random perturbation of F alone moves this update
+        # by 1e-9 .. 6e-9 (S and the solve are bit-reproducible).  FP32 later
+        # epochs: the recovered warm start differs from the device's by
+        # rounding (~1e-12 relative), which 6 CG steps of the ill-conditioned
+        # symmetric system amplify further: stated bound 1e-4
+        assert err <= (1e-9 if prec == 64 else 1e-8 if ep == 0 else 1e-4), (ep, err)
     assert replays >= 2
 
 
