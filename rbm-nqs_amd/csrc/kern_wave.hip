@@ -3,17 +3,17 @@
 
 namespace rbmgpu {
 
-template <int K, bool SYM>
+template <int K, bool SYM, bool PAIRS>
 static hipError_t wave_one(const SampArgs<float>& a, int block, hipStream_t st) {
     const int waves = block / 64;
     const int grid = (a.n_chains + waves - 1) / waves;
     const size_t lds = wave_lds_bytes(a.d, K, a.npair);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sample_wave<K, SYM>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_sample_wave<K, SYM, PAIRS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sample_wave<K, SYM>), dim3(grid), dim3(block), lds, st, a);
+    hipLaunchKernelGGL((k_sample_wave<K, SYM, PAIRS>), dim3(grid), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
@@ -27,7 +27,9 @@ hipError_t launch_sample_wave(const SampArgs<float>& a, int K, int block, hipStr
     if (sym ? 64 % a.d.n_alpha != 0 : a.npair != 0) return hipErrorInvalidValue;
 #define WAVE_K(K_)                                                               \
     case K_:                                                                     \
-        return sym ? wave_one<K_, true>(a, block, st) : wave_one<K_, false>(a, block, st);
+        return !sym           ? wave_one<K_, false, false>(a, block, st)      \
+               : a.npair > 0 ? wave_one<K_, true, true>(a, block, st)        \
+                             : wave_one<K_, true, false>(a, block, st);
     switch (K) {
         WAVE_K(1)
         WAVE_K(2)

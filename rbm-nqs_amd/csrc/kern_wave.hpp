@@ -189,6 +189,7 @@ template <int K, bool SYM>
 struct WaveUnits {
     cplx<float> th[K], tn[K];
     cplx<float> rr[K];  // the last proposal's unit factors (the Mobius step's denominators)
+    int en[K];          // and their table entries
     int pbase, mbase, aoff, aoff2, ln, lg;  // lg = log2(alpha_n) (symmetric), aoff2 = 2 aoff
     int ustride, mstride;  // symmetric: byte strides of one unit in perm / pair map
     bool vlast;            // the lane's last unit is a real one
@@ -257,7 +258,8 @@ struct WaveUnits {
         float lsum = 0.f, prod = 1.f;
 #pragma unroll
         for (int u = 0; u < K; u++) {
-            const cplx<float> r = factor1(u, tab[e(u)], ns0);
+            en[u] = e(u);
+            const cplx<float> r = factor1(u, tab[en[u]], ns0);
             rr[u] = r;
             float n = r.re * r.re + r.im * r.im;
             if (u == K - 1 && !vlast) n = 1.f;
@@ -304,12 +306,13 @@ struct WaveUnits {
         const float inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
         tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
     }
-    template <class E>
-    __device__ void accept1(const float4* tab, const float2* wt, E e, float s0) {
+    // accepted move of the entries lratio1 kept: single flip (tab = CS,
+    // wt = W) or pair-table bond flip (PCS, PW)
+    __device__ void accept1(const float4* tab, const float2* wt, float s0) {
         const float ns0 = -s0, t2 = -2.f * s0;
 #pragma unroll
         for (int u = 0; u < K; u++) {
-            const int eu = e(u);
+            const int eu = en[u];
             const float4 cs = tab[eu];
             const float2 w = wt[eu];
             th[u] = {fmaf(t2, w.x, th[u].re), fmaf(t2, w.y, th[u].im)};
@@ -511,7 +514,11 @@ __device__ inline void wave_obs(const Dims& d, const WaveLds& l, const WaveUnits
     }
 }
 
-template <int K, bool SYM>
+// PAIRS (symmetric RBM with pair tables, npair > 0): every proposal's ratio
+// and accepted move run through one table entry per unit, so the accept
+// path is a single block (old and new th / tn never live side by side,
+// which keeps the loop-carried registers in place)
+template <int K, bool SYM, bool PAIRS>
 __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
     extern __shared__ __align__(16) char smem[];
     const Dims d = a.d;
@@ -538,7 +545,7 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
     WaveUnits<K, SYM> U;
     U.init(d, ln);
     U.from_state(d, l, a.t.hb, st0, st1);
-    const bool pairs = SYM && a.npair > 0;
+    static_assert(SYM || !PAIRS, "pair tables belong to the symmetric RBM");
 
     // proposals of slots sw0 .. sw0 + 63 are drawn ahead, one Philox block
     // per lane (lane ln: slot sw0 + ln): code = f0 | (f1 + 1) << 8 | b << 16
@@ -562,13 +569,12 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
     uint32_t acc = 0, prop = 0;
     int64_t k = 0;
     for (int64_t step = 0; step < total; step++) {
-        int sw0 = 0;
-        draw(step, 0);
-        for (int sw = 0; sw < d.N; sw++) {
-            if (sw >= sw0 + 64) {
-                sw0 = sw & ~63;
-                draw(step, sw0);
-            }
+        // slots sw0 .. sw0 + 63 are drawn before their proposals run; a pair
+        // takes two slots, so a chunk's last pair may step past its end
+        for (int sw = 0; sw < d.N;) {
+          const int sw0 = sw & ~63, swe = min(sw0 + 64, d.N);
+          draw(step, sw0);
+          for (; sw < swe; sw++) {
             const int code = __builtin_amdgcn_readlane(pcode, sw - sw0);
             const float lu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(plu), sw - sw0));
             const int f0 = code & 0xFF, f1 = ((code >> 8) & 0xFF) - 1, b = code >> 16;
@@ -585,7 +591,7 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
                 sw++;
                 const float s1 = wbit(st0, st1, f1) ? -1.f : 1.f;
                 bre += -2.f * s1 * rfl(l.rb[f1].x);
-                if (pairs) {
+                if constexpr (PAIRS) {
                     lr = U.lratio1(l.PCS, ep, -s0);
                 } else {
                     lr = U.lratio2(d, l, f0, f1, -s0, -s1);
@@ -594,10 +600,9 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
             // log2 |psi'/psi|^2 = wave sum of the lanes' logs + 2 Re(bias) / ln 2
             const float lr2 = wave_sum(lr) + 2.8853900817779268f * bre;
             if (lu < lr2) {
-                if (f1 < 0) {
-                    U.accept1(l.CS, l.W, e1, s0);
-                } else if (pairs) {
-                    U.accept1(l.PCS, l.PW, ep, s0);
+                if (PAIRS || f1 < 0) {
+                    const bool one = f1 < 0;
+                    U.accept1(one ? l.CS : l.PCS, one ? l.W : l.PW, s0);
                 } else {
                     const float s1 = wbit(st0, st1, f1) ? -1.f : 1.f;
                     U.accept2(d, l, f0, f1, s0, s1);
@@ -610,6 +615,7 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
                 }
                 acc++;
             }
+          }
         }
         U.resync();
         if (step >= a.warmup && (step - a.warmup) % a.step == 0) {
