@@ -136,16 +136,16 @@ __device__ inline WaveLds wave_stage(const SampArgs<float>& a, char* smem) {
     if (d.n_symm > 1) {
         uint8_t* P = reinterpret_cast<uint8_t*>(at);
         for (int i = threadIdx.x; i < nsp * d.N; i += blockDim.x) {
-            const int s = i / d.N;
-            P[i] = a.t.perm[(s < d.n_symm ? i : i - s * d.N)];
+            const int sy = i / d.N;
+            P[i] = a.t.perm[(sy < d.n_symm ? i : i - sy * d.N)];
         }
         at += (size_t)nsp * d.N;
         l.P = P;
         if (a.npair) {
             uint8_t* PM = reinterpret_cast<uint8_t*>(at);
             for (int i = threadIdx.x; i < nsp * d.npb; i += blockDim.x) {
-                const int s = i / d.npb;
-                PM[i] = a.pmap[(s < d.n_symm ? i : i - s * d.npb)];
+                const int sy = i / d.npb;
+                PM[i] = a.pmap[(sy < d.n_symm ? i : i - sy * d.npb)];
             }
             at += (size_t)nsp * d.npb;
             l.PM = PM;
@@ -156,6 +156,10 @@ __device__ inline WaveLds wave_stage(const SampArgs<float>& a, char* smem) {
 }
 
 __device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// -1 for a set spin bit, +1 otherwise, and v with its sign flipped by a set
+// bit, as integer operations (scalar registers for wave-uniform bits)
+__device__ inline float sgn_of(int bit) { return __int_as_float(bit ? (int)0xBF800000 : 0x3F800000); }
+__device__ inline float flip_if(float v, int bit) { return __int_as_float(__float_as_int(v) ^ (bit << 31)); }
 __device__ inline float rfl(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 __device__ inline double rfl(double v) {
     int2 h = *reinterpret_cast<int2*>(&v);
@@ -189,6 +193,7 @@ template <int K, bool SYM>
 struct WaveUnits {
     cplx<float> th[K], tn[K];
     cplx<float> rr[K];  // the last proposal's unit factors (the Mobius step's denominators)
+    float rn[K];        // their squared moduli
     int en[K];          // and their table entries
     int pbase, mbase, aoff, aoff2, ln, lg;  // lg = log2(alpha_n) (symmetric), aoff2 = 2 aoff
     int ustride, mstride;  // symmetric: byte strides of one unit in perm / pair map
@@ -262,6 +267,7 @@ struct WaveUnits {
             const cplx<float> r = factor1(u, tab[en[u]], ns0);
             rr[u] = r;
             float n = r.re * r.re + r.im * r.im;
+            rn[u] = n;
             if (u == K - 1 && !vlast) n = 1.f;
             prod *= n;
             if ((u & 3) == 3 || u == K - 1) {
@@ -288,6 +294,7 @@ struct WaveUnits {
             const cplx<float> r = ch + tn[u] * sh;
             rr[u] = r;
             float n = r.re * r.re + r.im * r.im;
+            rn[u] = n;
             if (u == K - 1 && !vlast) n = 1.f;
             prod *= n;
             if ((u & 3) == 3 || u == K - 1) {
@@ -301,11 +308,13 @@ struct WaveUnits {
     // step tanh(theta + delta) = (t ch + sh) / r with r = ch + t sh, the
     // proposal's unit factor kept in rr (resync() restores tanh(theta) after
     // every sweep)
-    __device__ void mobius(int u, cplx<float> ch, cplx<float> sh, cplx<float> r) {
-        const cplx<float> num = tn[u] * ch + sh;
-        const float inv = __builtin_amdgcn_rcpf(r.re * r.re + r.im * r.im);
+    // (the squared modulus of r kept from the ratio)
+    __device__ void mobius_num(int u, cplx<float> num) {
+        const cplx<float> r = rr[u];
+        const float inv = __builtin_amdgcn_rcpf(rn[u]);
         tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
     }
+    __device__ void mobius(int u, cplx<float> ch, cplx<float> sh) { mobius_num(u, tn[u] * ch + sh); }
     // accepted move of the entries lratio1 kept: single flip (tab = CS,
     // wt = W) or pair-table bond flip (PCS, PW)
     __device__ void accept1(const float4* tab, const float2* wt, float s0) {
@@ -316,7 +325,10 @@ struct WaveUnits {
             const float4 cs = tab[eu];
             const float2 w = wt[eu];
             th[u] = {fmaf(t2, w.x, th[u].re), fmaf(t2, w.y, th[u].im)};
-            mobius(u, {cs.x, cs.y}, {ns0 * cs.z, ns0 * cs.w}, rr[u]);
+            // tanh(theta) C + ns0 S
+            const cplx<float> t = tn[u];
+            mobius_num(u, {fmaf(ns0, cs.z, fmaf(t.re, cs.x, -t.im * cs.y)),
+                           fmaf(ns0, cs.w, fmaf(t.re, cs.y, t.im * cs.x))});
         }
     }
     __device__ void accept2(const Dims& d, const WaveLds& l, int f0, int f1, float s0, float s1) {
@@ -327,7 +339,7 @@ struct WaveUnits {
             th[u] = {th[u].re - 2.f * (s0 * w0.x + s1 * w1.x), th[u].im - 2.f * (s0 * w0.y + s1 * w1.y)};
             cplx<float> ch, sh;
             compose2(l, e0, e1, -s0, -s1, ch, sh);
-            mobius(u, ch, sh, rr[u]);
+            mobius(u, ch, sh);
         }
     }
 };
@@ -579,33 +591,37 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
             const float lu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(plu), sw - sw0));
             const int f0 = code & 0xFF, f1 = ((code >> 8) & 0xFF) - 1, b = code >> 16;
             prop++;
-            const float s0 = wbit(st0, st1, f0) ? -1.f : 1.f;
-            float bre = -2.f * s0 * rfl(l.rb[f0].x);
+            // the proposal's spins and bias term are wave-uniform: integer
+            // selects and sign flips keep them in scalar registers
+            const int bit0 = wbit(st0, st1, f0) ? 1 : 0;
+            const int bit1 = f1 >= 0 && wbit(st0, st1, f1) ? 1 : 0;
+            const float s0 = sgn_of(bit0);
+            float bre = flip_if(rfl(l.rb[f0].x), bit0);  // s0 Re b_f0
             float lr;
-            const int combo = f1 >= 0 && s0 != (wbit(st0, st1, f1) ? -1.f : 1.f) ? 1 : 0;
+            const int combo = f1 >= 0 ? bit0 ^ bit1 : 0;
             auto e1 = [&](int u) __attribute__((always_inline)) { return U.entry(d, l, u, f0); };
             auto ep = [&](int u) __attribute__((always_inline)) { return U.pentry(d, l, u, b, combo); };
             if (f1 < 0) {
                 lr = U.lratio1(l.CS, e1, -s0);
             } else {
                 sw++;
-                const float s1 = wbit(st0, st1, f1) ? -1.f : 1.f;
-                bre += -2.f * s1 * rfl(l.rb[f1].x);
+                const float s1 = sgn_of(bit1);
+                bre += flip_if(rfl(l.rb[f1].x), bit1);
                 if constexpr (PAIRS) {
                     lr = U.lratio1(l.PCS, ep, -s0);
                 } else {
                     lr = U.lratio2(d, l, f0, f1, -s0, -s1);
                 }
             }
-            // log2 |psi'/psi|^2 = wave sum of the lanes' logs + 2 Re(bias) / ln 2
-            const float lr2 = wave_sum(lr) + 2.8853900817779268f * bre;
+            // log2 |psi'/psi|^2 = wave sum of the lanes' logs + 2 Re(bias
+            // change) / ln 2, the bias change -2 sum s_f Re b_f = -2 bre
+            const float lr2 = wave_sum(lr) - 5.7707801635558536f * bre;
             if (lu < lr2) {
                 if (PAIRS || f1 < 0) {
                     const bool one = f1 < 0;
                     U.accept1(one ? l.CS : l.PCS, one ? l.W : l.PW, s0);
                 } else {
-                    const float s1 = wbit(st0, st1, f1) ? -1.f : 1.f;
-                    U.accept2(d, l, f0, f1, s0, s1);
+                    U.accept2(d, l, f0, f1, s0, sgn_of(bit1));
                 }
                 if (f0 < 64) st0 ^= (uint64_t)1 << f0;
                 else st1 ^= (uint64_t)1 << (f0 - 64);
