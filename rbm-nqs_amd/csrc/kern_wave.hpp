@@ -32,7 +32,7 @@ namespace rbmgpu {
 
 struct WaveLds {
     const float4* CS;   // (C, S) of entry e = site * alpha_n + alpha (basic: site * H + h)
-    const float4* PCS;  // pair tables: ((pair * alpha_n + alpha) * 2 + combo)
+    const float4* PCS;  // pair tables: combo * npair * alpha_n + pair * alpha_n + alpha
     const float2* W;
     const float2* PW;
     const float2* rb;
@@ -77,10 +77,11 @@ __device__ inline WaveLds wave_stage(const SampArgs<float>& a, char* smem) {
         const int e0 = a.pairs[2 * p] * na + al, e1 = a.pairs[2 * p + 1] * na + al;
         const cplx<float> c0 = C[e0], s0 = S[e0], c1 = C[e1], s1 = S[e1];
         const cplx<float> cc = c0 * c1, ss = s0 * s1, sc = s0 * c1, cs = c0 * s1;
-        PCS[2 * i] = make_float4(cc.re + ss.re, cc.im + ss.im, sc.re + cs.re, sc.im + cs.im);
-        PCS[2 * i + 1] = make_float4(cc.re - ss.re, cc.im - ss.im, sc.re - cs.re, sc.im - cs.im);
-        PW[2 * i] = make_float2(Wg[e0].re + Wg[e1].re, Wg[e0].im + Wg[e1].im);
-        PW[2 * i + 1] = make_float2(Wg[e0].re - Wg[e1].re, Wg[e0].im - Wg[e1].im);
+        const int i1 = a.npair * na + i;  // combo 1 half
+        PCS[i] = make_float4(cc.re + ss.re, cc.im + ss.im, sc.re + cs.re, sc.im + cs.im);
+        PCS[i1] = make_float4(cc.re - ss.re, cc.im - ss.im, sc.re - cs.re, sc.im - cs.im);
+        PW[i] = make_float2(Wg[e0].re + Wg[e1].re, Wg[e0].im + Wg[e1].im);
+        PW[i1] = make_float2(Wg[e0].re - Wg[e1].re, Wg[e0].im - Wg[e1].im);
     }
     l.CS = CS;
     l.PCS = PCS;
@@ -195,17 +196,18 @@ struct WaveUnits {
     cplx<float> rr[K];  // the last proposal's unit factors (the Mobius step's denominators)
     float rn[K];        // their squared moduli
     int en[K];          // and their table entries
-    int pbase, mbase, aoff, aoff2, ln, lg;  // lg = log2(alpha_n) (symmetric), aoff2 = 2 aoff
+    int pbase, mbase, aoff, npa, ln, lg;  // lg = log2(alpha_n) (symmetric), npa = npair alpha_n
     int ustride, mstride;  // symmetric: byte strides of one unit in perm / pair map
     bool vlast;            // the lane's last unit is a real one
 
-    __device__ void init(const Dims& d, int ln_) {
+    __device__ void init(const Dims& d, int ln_, int npair) {
         ln = ln_;
+        npa = 0;
         if constexpr (SYM) {
             const int na = d.n_alpha, g = 64 / na;
+            npa = npair * na;
             lg = __builtin_ctz(na);
             aoff = ln & (na - 1);
-            aoff2 = 2 * aoff;
             const int sl = ln / na;
             pbase = sl * d.N;
             mbase = sl * d.npb;
@@ -213,7 +215,7 @@ struct WaveUnits {
             mstride = g * d.npb;
             vlast = sl + (K - 1) * g < d.n_symm;
         } else {
-            aoff = aoff2 = lg = pbase = mbase = ustride = mstride = 0;
+            aoff = lg = pbase = mbase = ustride = mstride = 0;
             vlast = ln + 64 * (K - 1) < d.H;
         }
     }
@@ -224,7 +226,7 @@ struct WaveUnits {
     }
     // pair-table entry of unit u for proposal bond b
     __device__ int pentry(const Dims& d, const WaveLds& l, int u, int b, int combo) const {
-        return ((int)l.PM[mbase + b + u * mstride] << (lg + 1)) + aoff2 + combo;
+        return ((int)l.PM[mbase + b + u * mstride] << lg) + (aoff + (combo ? npa : 0));
     }
     __device__ cplx<float> hbias(const Dims& d, const cplx<float>* hb, int u) const {
         if constexpr (SYM) return hb[aoff];
@@ -555,7 +557,7 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
         else if (d.N < 128) st1 &= (((uint64_t)1) << (d.N - 64)) - 1;
     }
     WaveUnits<K, SYM> U;
-    U.init(d, ln);
+    U.init(d, ln, a.npair);
     U.from_state(d, l, a.t.hb, st0, st1);
     static_assert(SYM || !PAIRS, "pair tables belong to the symmetric RBM");
 
