@@ -157,6 +157,20 @@ __device__ inline WaveLds wave_stage(const SampArgs<float>& a, char* smem) {
 }
 
 __device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// LDS byte address of an LDS pointer, and loads at LDS byte addresses
+__device__ inline uint32_t lds_addr(const void* p) {
+    return (uint32_t)(size_t)(__attribute__((address_space(3))) const char*)p;
+}
+typedef float lds_v4f __attribute__((ext_vector_type(4)));
+typedef float lds_v2f __attribute__((ext_vector_type(2)));
+__device__ inline float4 lds_f4(uint32_t a) {
+    const lds_v4f v = *(__attribute__((address_space(3))) const lds_v4f*)(size_t)a;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float2 lds_f2(uint32_t a) {
+    const lds_v2f v = *(__attribute__((address_space(3))) const lds_v2f*)(size_t)a;
+    return make_float2(v.x, v.y);
+}
 // -1 for a set spin bit, +1 otherwise, and v with its sign flipped by a set
 // bit, as integer operations (scalar registers for wave-uniform bits)
 __device__ inline float sgn_of(int bit) { return __int_as_float(bit ? (int)0xBF800000 : 0x3F800000); }
@@ -195,12 +209,13 @@ struct WaveUnits {
     cplx<float> th[K], tn[K];
     cplx<float> rr[K];  // the last proposal's unit factors (the Mobius step's denominators)
     float rn[K];        // their squared moduli
-    int en[K];          // and their table entries
+    uint32_t ea[K];     // and the LDS byte addresses of their (C, S) table records
     int pbase, mbase, aoff, npa, ln, lg;  // lg = log2(alpha_n) (symmetric), npa = npair alpha_n
+    uint32_t lcs, lpcs;  // LDS byte addresses of this lane's first CS / PCS record (alpha aoff)
     int ustride, mstride;  // symmetric: byte strides of one unit in perm / pair map
     bool vlast;            // the lane's last unit is a real one
 
-    __device__ void init(const Dims& d, int ln_, int npair) {
+    __device__ void init(const Dims& d, const WaveLds& l, int ln_, int npair) {
         ln = ln_;
         npa = 0;
         if constexpr (SYM) {
@@ -218,6 +233,18 @@ struct WaveUnits {
             aoff = lg = pbase = mbase = ustride = mstride = 0;
             vlast = ln + 64 * (K - 1) < d.H;
         }
+        lcs = lds_addr(l.CS + aoff);
+        lpcs = lds_addr(l.PCS + aoff);
+    }
+    // LDS byte address of the (C, S) record of unit u: single flip of site f
+    // (symmetric RBM: one shift-add from the permutation byte), pair-table
+    // bond flip b
+    __device__ uint32_t entry_a(const Dims& d, const WaveLds& l, int u, int f) const {
+        if constexpr (SYM) return ((uint32_t)l.P[pbase + f + u * ustride] << (lg + 4)) + lcs;
+        else return lds_addr(l.CS + entry(d, l, u, f));
+    }
+    __device__ uint32_t pentry_a(const Dims& d, const WaveLds& l, int u, int b, int combo) const {
+        return ((uint32_t)l.PM[mbase + b + u * mstride] << (lg + 4)) + (lpcs + (combo ? 16u * npa : 0u));
     }
     // single-site entry of unit u for site f
     __device__ int entry(const Dims& d, const WaveLds& l, int u, int f) const {
@@ -261,12 +288,12 @@ struct WaveUnits {
     // log2 of the lane's prod_u |C + tanh (ns0 S)|^2 for one table entry per
     // unit (single flip, or a pair entry)
     template <class E>
-    __device__ float lratio1(const float4* tab, E e, float ns0) {
+    __device__ float lratio1(E e, float ns0) {
         float lsum = 0.f, prod = 1.f;
 #pragma unroll
         for (int u = 0; u < K; u++) {
-            en[u] = e(u);
-            const cplx<float> r = factor1(u, tab[en[u]], ns0);
+            ea[u] = e(u);
+            const cplx<float> r = factor1(u, lds_f4(ea[u]), ns0);
             rr[u] = r;
             float n = r.re * r.re + r.im * r.im;
             rn[u] = n;
@@ -317,15 +344,15 @@ struct WaveUnits {
         tn[u] = {(num.re * r.re + num.im * r.im) * inv, (num.im * r.re - num.re * r.im) * inv};
     }
     __device__ void mobius(int u, cplx<float> ch, cplx<float> sh) { mobius_num(u, tn[u] * ch + sh); }
-    // accepted move of the entries lratio1 kept: single flip (tab = CS,
-    // wt = W) or pair-table bond flip (PCS, PW)
-    __device__ void accept1(const float4* tab, const float2* wt, float s0) {
+    // accepted move of the records lratio1 kept: single flip or pair-table
+    // bond flip, the W record at (CS address) / 2 + wofs (the W tables hold
+    // 8-byte records in the CS tables' order)
+    __device__ void accept1(uint32_t wofs, float s0) {
         const float ns0 = -s0, t2 = -2.f * s0;
 #pragma unroll
         for (int u = 0; u < K; u++) {
-            const int eu = en[u];
-            const float4 cs = tab[eu];
-            const float2 w = wt[eu];
+            const float4 cs = lds_f4(ea[u]);
+            const float2 w = lds_f2((ea[u] >> 1) + wofs);
             th[u] = {fmaf(t2, w.x, th[u].re), fmaf(t2, w.y, th[u].im)};
             // tanh(theta) C + ns0 S
             const cplx<float> t = tn[u];
@@ -557,7 +584,9 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
         else if (d.N < 128) st1 &= (((uint64_t)1) << (d.N - 64)) - 1;
     }
     WaveUnits<K, SYM> U;
-    U.init(d, ln, a.npair);
+    U.init(d, l, ln, a.npair);
+    // W record address = (CS record address) / 2 + offset, per table
+    const uint32_t wofs1 = lds_addr(l.W) - lds_addr(l.CS) / 2, wofs2 = lds_addr(l.PW) - lds_addr(l.PCS) / 2;
     U.from_state(d, l, a.t.hb, st0, st1);
     static_assert(SYM || !PAIRS, "pair tables belong to the symmetric RBM");
 
@@ -601,16 +630,16 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
             float bre = flip_if(rfl(l.rb[f0].x), bit0);  // s0 Re b_f0
             float lr;
             const int combo = f1 >= 0 ? bit0 ^ bit1 : 0;
-            auto e1 = [&](int u) __attribute__((always_inline)) { return U.entry(d, l, u, f0); };
-            auto ep = [&](int u) __attribute__((always_inline)) { return U.pentry(d, l, u, b, combo); };
+            auto e1 = [&](int u) __attribute__((always_inline)) { return U.entry_a(d, l, u, f0); };
+            auto ep = [&](int u) __attribute__((always_inline)) { return U.pentry_a(d, l, u, b, combo); };
             if (f1 < 0) {
-                lr = U.lratio1(l.CS, e1, -s0);
+                lr = U.lratio1(e1, -s0);
             } else {
                 sw++;
                 const float s1 = sgn_of(bit1);
                 bre += flip_if(rfl(l.rb[f1].x), bit1);
                 if constexpr (PAIRS) {
-                    lr = U.lratio1(l.PCS, ep, -s0);
+                    lr = U.lratio1(ep, -s0);
                 } else {
                     lr = U.lratio2(d, l, f0, f1, -s0, -s1);
                 }
@@ -620,8 +649,7 @@ __global__ void __launch_bounds__(1024) k_sample_wave(SampArgs<float> a) {
             const float lr2 = wave_sum(lr) - 5.7707801635558536f * bre;
             if (lu < lr2) {
                 if (PAIRS || f1 < 0) {
-                    const bool one = f1 < 0;
-                    U.accept1(one ? l.CS : l.PCS, one ? l.W : l.PW, s0);
+                    U.accept1(f1 < 0 ? wofs1 : wofs2, s0);
                 } else {
                     U.accept2(d, l, f0, f1, s0, sgn_of(bit1));
                 }
