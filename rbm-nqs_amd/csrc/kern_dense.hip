@@ -281,44 +281,10 @@ __global__ void __launch_bounds__(256) k_s_from_g(SyrkArgs<R> a, int P, int n_ne
     Sm[2 * t + 1] = im;
 }
 
-// y = S x for the Hermitian S (column-major): y_i = sum_j conj(S_ji) x_j, so
-// wave i reads column i contiguously; per block the partial x^H y of its
-// rows for the CG step.  4 rows per block.
 __global__ void __launch_bounds__(256) k_dense_mv(const double* Sm, const double* x, double* y, int P, double* pdot,
                                                   const CgStatus* st) {
     if (st && st->done) return;
-    __shared__ double red[2][4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = blockIdx.x * 4 + w;
-    double yr = 0, yi = 0;
-    if (i < P) {
-        const double* col = Sm + 2 * (int64_t)i * P;
-        for (int j = lane; j < P; j += 64) {
-            const double sr = col[2 * j], si = -col[2 * j + 1], xr = x[2 * j], xi = x[2 * j + 1];
-            yr += sr * xr - si * xi;
-            yi += sr * xi + si * xr;
-        }
-    }
-    yr = group_sum<64>(yr);
-    yi = group_sum<64>(yi);
-    double dr = 0, di = 0;
-    if (i < P) {
-        if (lane == 0) {
-            y[2 * i] = yr;
-            y[2 * i + 1] = yi;
-        }
-        const double xr = x[2 * i], xi = x[2 * i + 1];
-        dr = xr * yr + xi * yi;  // conj(x_i) y_i
-        di = xr * yi - xi * yr;
-    }
-    if (lane == 0) {
-        red[0][w] = dr;
-        red[1][w] = di;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && pdot) {
-        pdot[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-        pdot[2 * blockIdx.x + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
-    }
+    dense_mv_rows(Sm, x, y, P, pdot);
 }
 
 template <typename R>

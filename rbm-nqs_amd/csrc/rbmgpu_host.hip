@@ -82,6 +82,7 @@ struct rbmgpu_ctx {
     bool moments_valid = false;
     // CG / S.x
     double *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_pdot = nullptr;
+    unsigned* d_cgcnt = nullptr;  // blocks done in a fused dense mat-vec + CG step (0 between launches)
     CgStatus* d_st = nullptr;
     void* d_sxpart = nullptr;
     double* d_sxpart2 = nullptr;
@@ -979,9 +980,11 @@ __global__ void __launch_bounds__(1024) k_cg_init(const double* b, const double*
 // all their loads are issued before the first reduction, so the step waits
 // on global memory once; larger P walks the rest twice.
 constexpr int kCgReg = 4;
-__global__ void __launch_bounds__(1024) k_cg_update(const double* pdot, int nb, double* x, double* r, double* p,
-                                                    const double* Ap, int P, CgStatus* st) {
-    if (st->done) return;
+// one CG step after y = A p (cg_solver.cpp:160-185): alpha from the nb
+// partials of p^H Ap, x += alpha p, r -= alpha Ap, convergence on |r|^2,
+// p = r + beta p; one block, any multiple of 64 threads
+__device__ void cg_update_body(const double* pdot, int nb, double* x, double* r, double* p, const double* Ap, int P,
+                               CgStatus* st) {
     const int nt = blockDim.x;
     double pv[kCgReg][2], qv[kCgReg][2], xv[kCgReg][2], rv[kCgReg][2];
 #pragma unroll
@@ -1061,6 +1064,30 @@ __global__ void __launch_bounds__(1024) k_cg_update(const double* pdot, int nb, 
             }
         }
     }
+}
+__global__ void __launch_bounds__(1024) k_cg_update(const double* pdot, int nb, double* x, double* r, double* p,
+                                                    const double* Ap, int P, CgStatus* st) {
+    if (st->done) return;
+    cg_update_body(pdot, nb, x, r, p, Ap, P, st);
+}
+// Ap = S p on the explicit S and the CG step in one launch: the block that
+// finishes its rows last (device-scope release fence, then a counter; the
+// counter returns to 0 for the next launch) runs the step.  Launches queued
+// past convergence return at once.
+__global__ void __launch_bounds__(256) k_dense_cg(const double* Sm, double* x, double* r, double* p, double* Ap, int P,
+                                                  double* pdot, CgStatus* st, unsigned* cnt) {
+    __shared__ int last;
+    if (st->done) return;
+    dense_mv_rows(Sm, p, Ap, P, pdot);
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's Ap rows and partial before the count
+        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every block's rows and partials are visible past the count
+    cg_update_body(pdot, gridDim.x, x, r, p, Ap, P, st);
+    if (threadIdx.x == 0) atomicExch(cnt, 0u);
 }
 
 // <O>, Var(O), <E O*> of the basic RBM from the structured sums:
@@ -1275,6 +1302,8 @@ int ensure_cg(rbmgpu_ctx* c) {
     const int64_t npd = std::max<int64_t>((c->P + kColBlock - 1) / kColBlock, (c->P + 3) / 4);
     if (int rc = dalloc(ctx, &c->d_pdot, npd * 2 * sizeof(double))) return rc;
     if (int rc = dalloc(ctx, &c->d_st, sizeof(CgStatus))) return rc;
+    if (int rc = dalloc(ctx, &c->d_cgcnt, sizeof(unsigned))) return rc;
+    HIPCK(hipMemsetAsync(c->d_cgcnt, 0, sizeof(unsigned), c->stream));  // not captured: allocation time
     return 0;
 }
 
@@ -1907,15 +1936,20 @@ struct DevArgsScope {
 // return at once)
 // one matrix-vector product of the SR epoch's CG: the regularised S.x
 // matrix-free, or the explicit S (train_dense); cg: the CG step follows
+// Ap = S p and the CG step on the explicit S, one fused launch
+int dense_cg_step(rbmgpu_ctx* c) {
+    rbmgpu_ctx* ctx = c;
+    hipLaunchKernelGGL(k_dense_cg, dim3((unsigned)((c->P + 3) / 4)), dim3(256), 0, c->stream, c->d_Smat, c->d_x,
+                       c->d_r, c->d_p, c->d_Ap, (int)c->P, c->d_pdot, c->d_st, c->d_cgcnt);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
 int train_mv(rbmgpu_ctx* c, const double* x, double* y, const CgStatus* st, bool cg) {
     rbmgpu_ctx* ctx = c;
     if (c->train_dense != 1) return sx(c, x, y, 0, 0, 0, st, cg ? c->d_pdot : nullptr, cg);
-    HIPCK(launch_dense_mv(c->d_Smat, x, y, (int)c->P, cg ? c->d_pdot : nullptr, st, c->stream));
-    if (cg) {
-        hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, c->d_pdot, (int)((c->P + 3) / 4), c->d_x,
-                           c->d_r, c->d_p, c->d_Ap, (int)c->P, c->d_st);
-        HIPCK(hipGetLastError());
-    }
+    if (cg) return dense_cg_step(c);
+    HIPCK(launch_dense_mv(c->d_Smat, x, y, (int)c->P, nullptr, st, c->stream));
     return 0;
 }
 
@@ -1952,8 +1986,9 @@ int train_more(rbmgpu_ctx* c, int64_t n_cg) {
 }
 
 // the SR epoch's CG on the explicit S when that is cheaper: one SYRK of O
-// (2 S (2P)^2 flops: MFMA at ~40 TFLOP/s in FP32, VALU at ~15 in FP64) and
-// then per iteration a P x P complex mat-vec (~10 us at these sizes)
+// (2 S (2P)^2 flops: MFMA at ~100 TFLOP/s of that count in FP32 -- the
+// upper-triangle tiles do about 2/3 of it --, VALU at ~15 in FP64) and then
+// per iteration a P x P complex mat-vec fused with the CG step (~8 us)
 // against a pass over O per iteration (S ld 8 or 16 bytes at ~6 TB/s plus
 // ~15 us of launches); the same trade as the reference's cg-dynamic
 // (cg_solver.cpp:31-46), with this device's rates
@@ -1961,8 +1996,8 @@ int decide_train_dense(const rbmgpu_ctx* c, int64_t max_iter) {
     if (c->structured) return 0;  // no O matrix
     const double S = (double)c->n_samples, P = (double)c->P, ld = (double)c->ld;
     const double rsz = c->prec == RBMGPU_FP32 ? 8.0 : 16.0;
-    const double syrk = 2.0 * S * (2 * P) * (2 * P) / (c->prec == RBMGPU_FP32 ? 40e12 : 15e12);
-    const double dense = syrk + (double)max_iter * (P * P * 16.0 / 4e12 + 10e-6);
+    const double syrk = 2.0 * S * (2 * P) * (2 * P) / (c->prec == RBMGPU_FP32 ? 100e12 : 15e12);
+    const double dense = syrk + (double)max_iter * (P * P * 16.0 / 4e12 + 8e-6);
     const double mfree = (double)max_iter * (S * ld * rsz / 6e12 + 15e-6);
     return dense < mfree ? 1 : 0;
 }
@@ -2800,16 +2835,11 @@ int rbmgpu_sr_solve_ex(rbmgpu_ctx* c, int32_t method, const double* b, double* x
     int64_t mi = std::min<int64_t>(2 * c->P, max_iter);
     if (mi == 0) mi = 2 * c->P;
     if (rtol == 0.) rtol = 1e-16;
-    const int nb_dense = (int)((c->P + 3) / 4);
     // y = A x (the regularised S), CG step appended when cg
     auto matvec = [&](const double* xin, double* yout, const CgStatus* st, bool cg) -> int {
         if (!dense) return sx(c, xin, yout, r1, r2, rd, st, cg ? c->d_pdot : nullptr, cg);
-        HIPCK(launch_dense_mv(c->d_Smat, xin, yout, (int)c->P, cg ? c->d_pdot : nullptr, st, c->stream));
-        if (cg) {
-            hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, c->stream, c->d_pdot, nb_dense, c->d_x, c->d_r,
-                               c->d_p, c->d_Ap, (int)c->P, c->d_st);
-            HIPCK(hipGetLastError());
-        }
+        if (cg) return dense_cg_step(c);  // xin = p, yout = Ap
+        HIPCK(launch_dense_mv(c->d_Smat, xin, yout, (int)c->P, nullptr, st, c->stream));
         return 0;
     };
     HIPCK(hipMemcpyAsync(c->d_b, b, c->P * 16, hipMemcpyHostToDevice, c->stream));
