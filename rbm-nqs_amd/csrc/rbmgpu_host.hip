@@ -679,8 +679,11 @@ int autotune(rbmgpu_ctx* c) {
         c->block = cand_block[0] ? cand_block[0] : pick_block(c, c->shape.L, c->n_chains);
         return tune_chains(c);
     }
-    // time a short epoch (2 samples per chain, 2 warmup sweeps) per shape
+    // time a short epoch per shape: 2 samples per chain after up to 30
+    // warm-up sweeps, so the per-block table staging and the kept samples'
+    // local observables weigh about as much as in the real epoch
     if (int rc = ensure_samples(c, 2 * (int64_t)c->n_chains)) return rc;
+    const int trial_warm = std::max(2, std::min(c->warmup, 30));
     float best_ms = 1e30f;
     for (size_t ci = 0; ci < cand.size(); ci++) {
         const Shape s = cand[ci];
@@ -689,8 +692,8 @@ int autotune(rbmgpu_ctx* c) {
         for (int rep = 0; rep < 2; rep++) {
             HIPCK(hipEventRecord(c->ev[10], c->stream));
             int64_t ns = 2 * (int64_t)c->n_chains;
-            int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, 0xFFFFFFF0u, s, block, ns, 2)
-                                            : run_sample<double>(c, 0xFFFFFFF0u, s, block, ns, 2);
+            int rc = c->prec == RBMGPU_FP32 ? run_sample<float>(c, 0xFFFFFFF0u, s, block, ns, trial_warm)
+                                            : run_sample<double>(c, 0xFFFFFFF0u, s, block, ns, trial_warm);
             if (rc) return rc;
             HIPCK(hipEventRecord(c->ev[11], c->stream));
             HIPCK(hipEventSynchronize(c->ev[11]));
