@@ -217,10 +217,10 @@ def sampler_roofline(g, m, chains, spc, warmup):
 
 
 # HBM bytes per sampler launch of the headline workload from two rocprofv3
-# --pmc runs of `bench.py --sampler-only 8` (FETCH_SIZE 721 KiB x2.0, then
-# WRITE_SIZE 299250 KiB x1.0, scaled like the k_struct figure below): the
-# kept states and tanh rows it writes
-SAMPLER_TRAFFIC_BYTES = int((721.19 * 2.0 + 299250.0) * 1024)
+# --pmc runs of `bench.py --sampler-only 8` (FETCH_SIZE 1042.6 KiB x2.0,
+# then WRITE_SIZE 299214.8 KiB x1.0, scaled like the k_struct figure below):
+# the kept states and tanh rows it writes
+SAMPLER_TRAFFIC_BYTES = int((1042.56 * 2.0 + 299214.81) * 1024)
 # FP32 FMA peak without packed instructions: 256 CUs x 4 SIMDs x 16 lanes x 2 x 2.4 GHz
 FP32_NONPACKED_TFLOPS = 78.6
 
