@@ -11,7 +11,8 @@
 // (sample range).  Blocks are mapped XCD-aware: the dispatcher hands block
 // b to XCD b % 8, and every tile of one K slice runs on the same XCD, so
 // the slice's O rows come from HBM into that XCD's L2 once and the other
-// tiles re-read them there.  FP32 on MFMA: 32-sample chunks of both panels
+// tiles re-read them there.  FP32 on MFMA: 16-sample chunks (20 KB of LDS
+// per block, 8 blocks per CU) of both panels
 // go through registers into a double-buffered LDS stage (the global loads
 // of chunk c + 1 are in flight while chunk c is multiplied); each of the 4
 // waves owns a 32 x 32 quarter of the tile as 2 x 2
@@ -34,7 +35,9 @@ namespace rbmgpu {
 typedef float v4f_d __attribute__((ext_vector_type(4)));
 
 constexpr int kGT = 64;     // G tile edge
-constexpr int kGChunk = 32; // samples per LDS chunk
+constexpr int kGChunk = 32; // samples per LDS chunk (FP64)
+constexpr int kFChunk = 16; // samples per LDS chunk (FP32; 20 KB of LDS per block)
+constexpr int kFRows = kFChunk / 16;  // chunk rows a thread stages
 constexpr int kGRow = 80;   // FP32 LDS row stride: the 4 rows one MFMA operand read spans sit 16 banks apart
 
 // tile pair q of the upper triangle (ti <= tj), row-major over ti
@@ -84,7 +87,7 @@ __device__ inline void syrk_stage_f64(const SyrkArgs<double>& a, double (*As)[kG
 // FP32 staging: thread t carries columns 4 (t % 16) .. +3 of both panels for
 // chunk rows t / 16 and t / 16 + 16
 struct SyrkFetch {
-    float4 a[2], b[2];
+    float4 a[kFRows], b[kFRows];
 };
 __device__ inline float4 centred4(const float* X, int64_t row, int col, int ncol, int nvalid, const float* m,
                                   float f) {
@@ -99,7 +102,7 @@ __device__ inline float4 centred4(const float* X, int64_t row, int col, int ncol
 __device__ inline void syrk_fetch(const SyrkArgs<float>& a, SyrkFetch& r, int ca, int cb, bool diag,
                                   const float* ma, const float* mb, int64_t kc, int64_t k1) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < kFRows; h++) {
         const int64_t k = kc + (threadIdx.x >> 4) + 16 * h;
         if (k < k1) {
             const int64_t row = k * a.ncol;
@@ -114,8 +117,8 @@ __device__ inline void syrk_fetch(const SyrkArgs<float>& a, SyrkFetch& r, int ca
 
 // FP32: MFMA, each wave a 32 x 32 quarter of the tile (2 x 2 MFMA tiles)
 __global__ void __launch_bounds__(256) k_syrk_f32(SyrkArgs<float> a) {
-    __shared__ __align__(16) float As[2][kGChunk][kGRow];
-    __shared__ __align__(16) float Bs[2][kGChunk][kGRow];
+    __shared__ __align__(16) float As[2][kFChunk][kGRow];
+    __shared__ __align__(16) float Bs[2][kFChunk][kGRow];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
     const int wi = w >> 1, wj = w & 1;
     int q, slice, ti, tj;
@@ -139,18 +142,18 @@ __global__ void __launch_bounds__(256) k_syrk_f32(SyrkArgs<float> a) {
     if (k0 < k1) {
         syrk_fetch(a, r, ca, cb, diag, ma, mb, k0, k1);
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < kFRows; h++) {
             *reinterpret_cast<float4*>(&As[0][(tid >> 4) + 16 * h][cq]) = r.a[h];
             *reinterpret_cast<float4*>(&Bs[0][(tid >> 4) + 16 * h][cq]) = r.b[h];
         }
     }
     __syncthreads();
     int buf = 0;
-    for (int64_t kc = k0; kc < k1; kc += kGChunk) {
-        const bool more = kc + kGChunk < k1;
-        if (more) syrk_fetch(a, r, ca, cb, diag, ma, mb, kc + kGChunk, k1);
+    for (int64_t kc = k0; kc < k1; kc += kFChunk) {
+        const bool more = kc + kFChunk < k1;
+        if (more) syrk_fetch(a, r, ca, cb, diag, ma, mb, kc + kFChunk, k1);
 #pragma unroll
-        for (int kk = 0; kk < kGChunk; kk += 4) {
+        for (int kk = 0; kk < kFChunk; kk += 4) {
             const float a0 = As[buf][kk + lq][wi * 32 + lr], a1 = As[buf][kk + lq][wi * 32 + 16 + lr];
             const float b0 = Bs[buf][kk + lq][wj * 32 + lr], b1 = Bs[buf][kk + lq][wj * 32 + 16 + lr];
             acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
@@ -161,7 +164,7 @@ __global__ void __launch_bounds__(256) k_syrk_f32(SyrkArgs<float> a) {
         if (more) {
             // the other buffer was last read before the previous barrier
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kFRows; h++) {
                 *reinterpret_cast<float4*>(&As[buf ^ 1][(tid >> 4) + 16 * h][cq]) = r.a[h];
                 *reinterpret_cast<float4*>(&Bs[buf ^ 1][(tid >> 4) + 16 * h][cq]) = r.b[h];
             }
