@@ -11,6 +11,10 @@ sample sets, through the C ABI:
 Tolerances: FP64 1e-10 of max |S| (S), 1e-9 (CG iterates, direct x);
 FP32 (O in FP32, FP32 MFMA accumulation within a sample slice) 1e-4 of
 max |S| and 1e-3 of max |x|.
+
+8200 samples exercise the SYRK's many-slice layout: 32 K slices (a
+multiple of 8, so each slice's tiles share one XCD), the last three of
+them empty, and the fused S.p + CG step over 63+ blocks.
 """
 import numpy as np
 import pytest
@@ -107,3 +111,26 @@ def test_direct_matches_numpy(name, prec):
     print(f"[gpu-test] direct solve error {err:.2e}", flush=True)
     assert err <= tol, err
     assert itn == 0
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_explicit_S_many_slices_matches_oracle(prec):
+    """8200 random states of the Kitaev 6x6 symmetric RBM: S with the CG
+    regularisation against the oracle's O, then 6 cg-single iterations
+    against the oracle's CG"""
+    m, o, g, rng, O, ref = _setup("kitaev6_sym", prec, S=8200)
+    g.reduce(want_o=False)
+    got = g.build_S(R1, R2, RD)
+    want = _S_ref(O, ref, m.n_neural, "cg")
+    tol = 1e-10 if prec == 64 else 1e-4
+    err = np.max(np.abs(got - want)) / np.abs(want).max()
+    print(f"[gpu-test] explicit S (8200 samples) error {err:.2e}", flush=True)
+    assert err <= tol, err
+    F = ref["eo"] - ref["mean"] * ref["o_mean"].conj()
+    x0 = np.zeros(m.P, np.complex128)
+    xg, itg, _ = g.sr_solve_ex(1, F, x0, R1, R2, RD, max_iter=6, rtol=1e-300)
+    xo, ito, _ = po.cg(O, ref["o_mean"], ref["o_var"], R1, R2, RD, m.n_neural, F, x0, 6, 1e-300)
+    assert itg == ito == 6
+    err = np.max(np.abs(xg - xo)) / np.abs(xo).max()
+    print(f"[gpu-test] cg-single (8200 samples) iterate error {err:.2e}", flush=True)
+    assert err <= (1e-9 if prec == 64 else 1e-3), err
