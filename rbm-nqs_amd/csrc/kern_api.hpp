@@ -181,12 +181,19 @@ struct LearnArgs {
     uint32_t* lacc;          // per chain: accepted / proposed learned moves
     uint32_t* lprop;
 };
+// small networks (hidden width and sites at most 64) are evaluated by each
+// chain's wavefront on its own (FP32 VALU dot products of the bf16 weights,
+// no block synchronisation); larger ones by the block's 8 chains together
+// on MFMA
+__host__ __device__ inline bool mlp_wave_fwd(int Hn, int N) { return Hn <= 64 && N <= 64; }
 // LDS bytes of the proposal network region of k_sample_mlp (after the
 // sampler tables)
 __host__ __device__ inline size_t mlp_lds_bytes(int Hn, int Np, int N, int kmax) {
     size_t b = 2 * (size_t)Hn * Np * 2 + ((size_t)Hn + Np) * 4;  // W1, W2 bf16, b1, b2
     b += 16 * (size_t)Np * 2 + 16 * (size_t)Hn * 2 + 2 * 16 * (size_t)Np * 4;  // Sg, Ha, Zt, Zc
     b += (size_t)N * kmax * 8;  // cyc, seedc, seedl
+    b = (b + 15) & ~(size_t)15;
+    if (mlp_wave_fwd(Hn, N)) b += 2 * (size_t)Hn * Np * 4 + 16 * (size_t)Hn * 4;  // W1^T, W2^T FP32, hidden rows
     return (b + 15) & ~(size_t)15;
 }
 // chains (wavefronts) per k_sample_mlp block; the MFMA tiles have 16 rows,

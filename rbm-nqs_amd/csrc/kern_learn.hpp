@@ -18,9 +18,11 @@
 // mixture of the two Metropolis-Hastings kernels stays valid); only the
 // learned slots evaluate the network, for all 8 chains at once with two
 // bf16 MFMA GEMMs (v_mfma_f32_16x16x16_bf16), H = S W1^T and Z = relu(H)
-// W2^T, and only they synchronise the block.  A chain's current logits go
-// stale when it accepts a plain move and are recomputed at the next learned
-// slot.  Logits are always computed from the state itself (no incremental
+// W2^T, and only they synchronise the block.  Small networks (hidden width
+// and sites at most 64) skip the batching: each wavefront evaluates its own
+// chain's network on the FP32 VALU and the block never synchronises in the
+// slot loop.  A chain's current logits go stale when it accepts a plain move
+// and are recomputed at the next learned slot.  Logits are always computed from the state itself (no incremental
 // hidden layer), so q is an exact function of the state and the chain is
 // exactly balanced for the network as evaluated.
 //
@@ -44,6 +46,7 @@ struct MlpLds {
     uint16_t *W1, *W2, *Sg, *Ha;
     float *b1, *b2, *Zt, *Zc;
     uint8_t *cyc, *seedc, *seedl;
+    float *W1t, *W2t, *Hw;  // per-wavefront forward: W1^T [site][h], W2^T [h][site] (FP32), hidden rows
 };
 
 // the network region after the sampler tables (layout of mlp_lds_bytes)
@@ -72,6 +75,20 @@ __device__ inline MlpLds mlp_stage(const LearnArgs& a, char* base) {
     m.seedc = reinterpret_cast<uint8_t*>(at);
     at += (size_t)N * kmax;
     m.seedl = reinterpret_cast<uint8_t*>(at);
+    at += (size_t)6 * N * kmax;
+    m.W1t = m.W2t = m.Hw = nullptr;
+    if (mlp_wave_fwd(Hn, N)) {
+        at = base + (((size_t)(at - base) + 15) & ~(size_t)15);
+        m.W1t = reinterpret_cast<float*>(at);
+        m.W2t = m.W1t + (size_t)Hn * Np;
+        m.Hw = m.W2t + (size_t)Hn * Np;
+        for (int i = threadIdx.x; i < Hn * Np; i += blockDim.x) {
+            const int h = i / Np, j = i - h * Np;  // W1 row h, W2 row j = i / Hn
+            m.W1t[j * Hn + h] = __uint_as_float((uint32_t)a.W1[i] << 16);
+            const int j2 = i / Hn, h2 = i - j2 * Hn;
+            m.W2t[h2 * Np + j2] = __uint_as_float((uint32_t)a.W2[i] << 16);
+        }
+    }
     for (int i = threadIdx.x; i < Hn * Np / 2; i += blockDim.x) {
         reinterpret_cast<uint32_t*>(m.W1)[i] = reinterpret_cast<const uint32_t*>(a.W1)[i];
         reinterpret_cast<uint32_t*>(m.W2)[i] = reinterpret_cast<const uint32_t*>(a.W2)[i];
@@ -117,6 +134,38 @@ __device__ inline void mlp_forward16(const MlpLds& m, int Hn, int Np) {
         for (int r = 0; r < 4; r++) m.Zt[(4 * lq + r) * Np + o] = acc[r] + m.b2[o];
     }
     __syncthreads();
+}
+
+// this wavefront's logits Zt[w] = W2 relu(W1 s + b1) + b2 for its chain's
+// state with the flips f0, f1 applied, on the FP32 VALU (the bf16 weights
+// as FP32): lane h sums its hidden unit over the sites, the hidden row goes
+// through LDS, lane j sums its logit over the hidden units; wavefront-local
+// (no block synchronisation)
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ inline void mlp_forward_wave(const MlpLds& m, int Hn, int Np, int N, const uint64_t* st, uint64_t f0,
+                                        uint64_t f1, int w, int lane) {
+    float* hw = m.Hw + w * Hn;
+    if (lane < Hn) {
+        float acc = m.b1[lane];
+        for (int i = 0; i < N; i++) {
+            const int b = bitof(st, i) ^ (int)(((i < 64 ? f0 : f1) >> (i & 63)) & 1);
+            const float wv = m.W1t[i * Hn + lane];
+            acc = b ? acc - wv : acc + wv;
+        }
+        hw[lane] = fmaxf(acc, 0.f);
+    }
+    wave_lds_sync();
+    if (lane < N) {
+        float acc = m.b2[lane];
+#pragma unroll 8
+        for (int h = 0; h < Hn; h++) acc = fmaf(m.W2t[h * Np + lane], hw[h], acc);
+        m.Zt[w * Np + lane] = acc;
+    }
+    wave_lds_sync();
 }
 
 // this wave's row of Sg from a state with the flips of mask applied
@@ -180,10 +229,16 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
     U.init(d, lane);
     U.from_state(d, l, a.t.hb, st);
     cplx<float> pv[16];
+    const bool wf = mlp_wave_fwd(Hn, N);  // per-wavefront forward passes (uniform)
     // logits of the initial states
-    mlp_put_state(m, Np, N, st, 0, 0, w, lane);
-    mlp_forward16(m, Hn, Np);
+    if (wf) {
+        mlp_forward_wave(m, Hn, Np, N, st, 0, 0, w, lane);
+    } else {
+        mlp_put_state(m, Np, N, st, 0, 0, w, lane);
+        mlp_forward16(m, Hn, Np);
+    }
     for (int j = lane; j < Np; j += 64) m.Zc[w * Np + j] = m.Zt[w * Np + j];
+    if (wf) wave_lds_sync();
     float lse = row_lse(m.Zt + w * Np, N, lane);
 
     // per-slot randomness, drawn ahead by the lanes (lane j: slot sw0 + j):
@@ -227,8 +282,9 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
                 sw0 = sw;
                 draw(step, sw0);
             }
-            const int code = __shfl(pcode, sw - sw0);
-            const float ua = __shfl(pua, sw - sw0), us = __shfl(pus, sw - sw0);
+            const int code = __builtin_amdgcn_readlane(pcode, sw - sw0);
+            const float ua = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pua), sw - sw0));
+            const float us = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pus), sw - sw0));
             const bool learned = (code & 0x40000000) != 0;
             int fl[8];
             float sg[8];
@@ -237,7 +293,15 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
             if (learned) {
                 // the block's chains whose current logits are stale (an
                 // accepted plain move since their last evaluation)
-                if (__syncthreads_or(stale ? 1 : 0)) {
+                if (wf) {
+                    if (stale) {
+                        mlp_forward_wave(m, Hn, Np, N, st, 0, 0, w, lane);
+                        for (int j = lane; j < Np; j += 64) m.Zc[w * Np + j] = m.Zt[w * Np + j];
+                        wave_lds_sync();
+                        lse = row_lse(m.Zt + w * Np, N, lane);
+                        stale = false;
+                    }
+                } else if (__syncthreads_or(stale ? 1 : 0)) {
                     mlp_put_state(m, Np, N, st, 0, 0, w, lane);
                     mlp_forward16(m, Hn, Np);
                     if (stale) {
@@ -301,8 +365,12 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
             const float* zt = m.Zt + w * Np;
             float lse2 = 0.f;
             if (learned) {
-                mlp_put_state(m, Np, N, st, fm0, fm1, w, lane);
-                mlp_forward16(m, Hn, Np);
+                if (wf) {
+                    mlp_forward_wave(m, Hn, Np, N, st, fm0, fm1, w, lane);
+                } else {
+                    mlp_put_state(m, Np, N, st, fm0, fm1, w, lane);
+                    mlp_forward16(m, Hn, Np);
+                }
                 lse2 = row_lse(zt, N, lane);
                 float qf = seed_mass(m, m.Zc + w * Np, lse, seed, kk, kmax);
                 float qr = seed_mass(m, zt, lse2, seed, kk, kmax);
@@ -320,6 +388,7 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
                 st[1] ^= fm1;
                 if (learned) {
                     for (int j = lane; j < Np; j += 64) m.Zc[w * Np + j] = zt[j];
+                    if (wf) wave_lds_sync();
                     lse = lse2;
                     lacc++;
                 } else {
@@ -356,7 +425,9 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
     }
 }
 
-// logits of host-given states through the same bf16 MFMA forward (8 per block)
+// logits of host-given states through the sampler's own forward (the
+// per-wavefront FP32 one for small networks, else the bf16 MFMA one; 8 per
+// block)
 __global__ void __launch_bounds__(64 * kMlpChains) k_mlp_logits(LearnArgs la, const uint64_t* states, int64_t n, float* z) {
     extern __shared__ __align__(16) char smem[];
     MlpLds m = mlp_stage(la, smem);
@@ -365,8 +436,12 @@ __global__ void __launch_bounds__(64 * kMlpChains) k_mlp_logits(LearnArgs la, co
     uint64_t st[2] = {0, 0};
     if (r < n)
         for (int q = 0; q < nw; q++) st[q] = states[r * nw + q];
-    mlp_put_state(m, la.Np, N, st, 0, 0, w, lane);
-    mlp_forward16(m, la.Hn, la.Np);
+    if (mlp_wave_fwd(la.Hn, N)) {
+        mlp_forward_wave(m, la.Hn, la.Np, N, st, 0, 0, w, lane);
+    } else {
+        mlp_put_state(m, la.Np, N, st, 0, 0, w, lane);
+        mlp_forward16(m, la.Hn, la.Np);
+    }
     if (r < n)
         for (int j = lane; j < N; j += 64) z[r * N + j] = m.Zt[w * la.Np + j];
 }

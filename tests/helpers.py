@@ -33,3 +33,13 @@ alpha = {alpha}
 def random_params(rng, P, std=0.1, std_imag=None):
     si = std if std_imag is None else std_imag
     return rng.normal(0, std, P) + 1j * rng.normal(0, si, P)
+
+
+def learn_net(rng, N, Hn, s1=0.4, s2=0.3):
+    """proposal network (W1, b1, W2, b2): W1 ~ s1 N(0, 1) (Hn x N), W2 ~ s2
+    N(0, 1) (N x Hn), biases 0.1 N(0, 1), drawn in that order"""
+    W1 = s1 * rng.normal(size=(Hn, N))
+    b1 = 0.1 * rng.normal(size=Hn)
+    W2 = s2 * rng.normal(size=(N, Hn))
+    b2 = 0.1 * rng.normal(size=N)
+    return W1, b1, W2, b2

@@ -4,16 +4,18 @@ checked against finite differences in test_oracle_learn.py, and exact
 enumeration for the sampled distribution).
 
 Tolerances: the training forward/backward runs in FP32 -> gradients within
-1e-4 of the largest gradient entry; the sampler's forward runs bf16 MFMA
-with bf16 hidden activations -> logits within 2e-2 of the logit scale
-against the oracle evaluated on the same bf16-rounded weights.
+1e-4 of the largest gradient entry; the sampler's forward of small
+networks (hidden width and sites at most 64) runs FP32 on the bf16 weights
+-> logits within 1e-4 of the logit scale, that of larger ones bf16 MFMA
+with bf16 hidden activations -> within 2e-2, both against the oracle
+evaluated on the same bf16-rounded weights.
 """
 import numpy as np
 import pytest
 
 import pyoracle as po
 import rbmgpu
-from helpers import ini, random_params
+from helpers import ini, learn_net, random_params
 
 pytestmark = pytest.mark.gpu
 
@@ -30,12 +32,7 @@ fraction = {fraction}
 """)
 
 
-def _net(rng, N, Hn, s1=0.4, s2=0.3):
-    W1 = s1 * rng.normal(size=(Hn, N))
-    b1 = 0.1 * rng.normal(size=Hn)
-    W2 = s2 * rng.normal(size=(N, Hn))
-    b2 = 0.1 * rng.normal(size=N)
-    return W1, b1, W2, b2
+_net = learn_net
 
 
 def _flat(W1, b1, W2, b2):
@@ -82,41 +79,62 @@ def test_learn_grad_matches_oracle(beta):
     assert np.max(np.abs(gg - ref)) <= 1e-4 * np.abs(ref).max(), np.max(np.abs(gg - ref)) / np.abs(ref).max()
 
 
-def test_learn_logits_bf16_mfma_matches_oracle():
-    """the sampler's bf16 MFMA forward (operand layouts of
-    v_mfma_f32_16x16x16_bf16) against the FP64 network on bf16 weights"""
-    m, g, rng = _gpu(learn_spec("kitaev", 3, 64, 0.5, 4, 64))
-    N, Hn = m.N, 64
+@pytest.mark.parametrize("Hn", [64, 80])
+def test_learn_logits_matches_oracle(Hn):
+    """the sampler's forward against the FP64 network on bf16 weights: at
+    hidden width 64 (sites and hidden units at most 64) the per-wavefront
+    FP32 forward, at 80 the bf16 MFMA one (operand layouts of
+    v_mfma_f32_16x16x16_bf16)"""
+    m, g, rng = _gpu(learn_spec("kitaev", 3, Hn, 0.5, 4, 64))
+    N = m.N
     net = _net(rng, N, Hn, 0.3, 0.2)
     g.learn_weights(_flat(*net))
     st = po.random_states(rng, 21, N)
     z = g.learn_logits(st)
     W1, b1, W2, b2 = net
     zo = po.mlp_logits(_bf16(W1), np.float32(b1), _bf16(W2), np.float32(b2), st, N)
-    assert np.max(np.abs(z - zo)) <= 2e-2 * (1 + np.abs(zo).max()), np.max(np.abs(z - zo))
+    tol = 1e-4 if Hn <= 64 else 2e-2
+    assert np.max(np.abs(z - zo)) <= tol * (1 + np.abs(zo).max()), np.max(np.abs(z - zo))
 
 
-@pytest.mark.parametrize("fraction", [0.6, 1.0])
-def test_learned_sampler_matches_exact_distribution(fraction):
+@pytest.mark.parametrize("fraction,scale", [(0.6, 0.8), (1.0, 0.3)])
+def test_learned_sampler_matches_exact_distribution(fraction, scale):
     """Metropolis-Hastings with the learned cluster proposal (untrained,
     non-uniform network) samples |psi|^2 exactly: <E> against exact
-    enumeration within 5 binned standard errors"""
+    enumeration within 5 binned standard errors, and the histogram of the
+    sampled states within total variation 0.04 of |psi|^2.
+
+    Learned moves alone (fraction 1) need a network whose chain mixes: with
+    the 0.8-scale network the exact transition matrix of the learned-only
+    chain has |lambda_2| = 1 - 2e-6 (relaxation ~4e5 slots), so no run of
+    this length reaches |psi|^2 on any device (the exact finite-run
+    distribution is 0.65 from it in total variation), while the 0.3-scale
+    one relaxes in ~70 slots (test_learned_chain_exact_balance_and_mixing).
+    At fraction 0.6 the plain single flips mix the 0.8-scale chain."""
     spec = learn_spec("kitaev", 2, 32, fraction, 256, 32768, warmup=50)
     m, g, rng = _gpu(spec, seed=5, std=0.4)
     o = po.Oracle(spec)
     p = random_params(np.random.default_rng(5), m.P, 0.4)
     o.set_params(p)
-    g.learn_weights(_flat(*_net(rng, m.N, 32, 0.8, 0.8)))
+    g.learn_weights(_flat(*_net(rng, m.N, 32, scale, scale)))
     exact = o.exact_expectation()
+    lp = o.logpsi(np.arange(1 << m.N, dtype=np.uint64).reshape(-1, 1))
+    pi = np.exp(2 * (lp.real - lp.real.max()))
+    pi /= pi.sum()
     means, stds = [], []
+    hist = np.zeros(1 << m.N)
     for ep in range(4):
         g.sample_epoch(epoch=ep)
         r = g.reduce(want_o=False)
         means.append(r["mean"].real)
         stds.append(r["std"])
+        st = g.get_samples(states=True, obs=False)[0]
+        hist += np.bincount(st[:, 0].astype(np.int64), minlength=1 << m.N)
     est = np.mean(means)
     err = np.sqrt(np.mean(np.square(stds)) / len(stds))
     assert abs(est - exact.real) < 5 * err + 1e-4, (est, exact, err)
+    tv = 0.5 * np.abs(hist / hist.sum() - pi).sum()
+    assert tv < 0.04, tv
 
 
 def test_learned_sampler_trains():

@@ -3,14 +3,15 @@ reference has no learned proposal): the FP64 proposal network and its
 batch gradient, and the cluster catalogue.
 
 Pins: central finite differences of the loss for every parameter block, the
-closed form of the loss at zero weights (uniform softmax), and the loops of
-the lattice (hexagons / vertex stars, pinned by the lattice tests).
+closed form of the loss at zero weights (uniform softmax), the loops of the
+lattice (hexagons / vertex stars, pinned by the lattice tests), and detailed
+balance of the exact transition matrix of the learned-only chain.
 """
 import numpy as np
 import pytest
 
 import pyoracle as po
-from helpers import ini
+from helpers import ini, learn_net, random_params
 
 
 def _net(rng, N, Hn, scale=0.3):
@@ -76,3 +77,64 @@ def test_cluster_catalogue(model, n, kmax):
     for i in range(o.N):
         assert tuple(sorted(cyc[i])) in loops
         assert len(set(cyc[i])) == kmax
+
+
+def _learned_chain(o, net, N, stale_reverse=False):
+    """exact transition matrix (2^N x 2^N) of the learned-only cluster chain
+    (fraction 1) and |psi|^2: seed s ~ softmax(z(x)), cluster size k ~
+    U{1..kmax}, flips = the first k sites of s's catalogue row; the forward
+    and reverse proposal probabilities sum softmax(z(x)), softmax(z(y)) over
+    every seed whose first k sites are the same set (the sampler's
+    seed_mass); stale_reverse=True plants the mistake of taking the reverse
+    one from the current state's logits z(x)"""
+    cyc = np.asarray(o.clusters())
+    kmax = cyc.shape[1]
+    S = np.arange(1 << N, dtype=np.uint64).reshape(-1, 1)
+    lp = o.logpsi(S)
+    pi = np.exp(2 * (lp.real - lp.real.max()))
+    pi /= pi.sum()
+    z = po.mlp_logits(*net, S, N)
+    q = np.exp(z - z.max(1, keepdims=True))
+    q /= q.sum(1, keepdims=True)
+    T = np.zeros((1 << N, 1 << N))
+    for s in range(N):
+        for k in range(1, kmax + 1):
+            cl = set(cyc[s, :k].tolist())
+            seeds = [j for j in range(N) if set(cyc[j, :k].tolist()) == cl]
+            m = sum(1 << f for f in cl)
+            for x in range(1 << N):
+                y = x ^ m
+                qf, qr = q[x, seeds].sum(), q[x if stale_reverse else y, seeds].sum()
+                a = min(1.0, pi[y] * qr / (pi[x] * qf))
+                T[x, y] += q[x, s] / kmax * a
+                T[x, x] += q[x, s] / kmax * (1 - a)
+    return T, pi
+
+
+@pytest.mark.parametrize("scale", [0.3, 0.8])
+def test_learned_chain_exact_balance_and_mixing(scale):
+    """Kitaev 2x2 (N = 8, 6-site hexagon clusters, so clusters of size 6
+    have 6 seeds), the networks of the GPU sampler test: the chain with the
+    cluster-mass proposal probabilities satisfies detailed balance exactly
+    (pi_x T_xy = pi_y T_yx); the planted variant that takes the reverse
+    proposal probability from the current state's logits violates it.  Mixing: |lambda_2| of the 0.3-scale
+    network's chain is below 0.99 (relaxation ~70 slots); that of the
+    0.8-scale one is above 1 - 1e-5, a chain that no finite run equilibrates"""
+    N = 8
+    o = po.Oracle(ini("kitaev", 2, extra="[learn]\nhidden = 32\nfraction = 1.0\n"))
+    rng = np.random.default_rng(5)
+    o.set_params(random_params(rng, len(o.get_params()), 0.4))
+    net = learn_net(rng, N, 32, scale, scale)
+    T, pi = _learned_chain(o, net, N)
+    assert np.allclose(T.sum(1), 1, atol=1e-13)
+    F = pi[:, None] * T
+    assert np.abs(F - F.T).max() < 1e-15
+    assert np.abs(pi @ T - pi).max() < 1e-14
+    lam2 = np.sort(np.abs(np.linalg.eigvals(T)))[::-1][1]
+    if scale == 0.3:
+        assert lam2 < 0.99, lam2
+    else:
+        assert lam2 > 1 - 1e-5, lam2
+    Tb, _ = _learned_chain(o, net, N, stale_reverse=True)
+    Fb = pi[:, None] * Tb
+    assert np.abs(Fb - Fb.T).max() > 1e-4
