@@ -197,8 +197,11 @@ __device__ inline float seed_mass(const MlpLds& m, const float* z, float lse, in
     return q;
 }
 
-template <int K>
-__global__ void __launch_bounds__(64 * kMlpChains) k_sample_mlp(LearnArgs la) {
+// WPE: minimum waves per SIMD the register allocation must allow (1: no
+// limit, the fastest code for one block per CU; 4: at most 128 VGPRs, so
+// that two blocks share a CU when there are more blocks than CUs)
+template <int K, int WPE>
+__global__ void __launch_bounds__(64 * kMlpChains) __attribute__((amdgpu_waves_per_eu(WPE))) k_sample_mlp(LearnArgs la) {
     extern __shared__ __align__(16) char smem[];
     const SampArgs<float>& a = la.s;
     const Dims d = a.d;
