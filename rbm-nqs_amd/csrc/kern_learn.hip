@@ -5,27 +5,33 @@
 namespace rbmgpu {
 
 template <int K, int WPE>
-static hipError_t mlp_launch(const LearnArgs& a, int grid, size_t lds, hipStream_t st) {
+static hipError_t mlp_launch(const LearnArgs& a, int grid, int cpb, size_t lds, hipStream_t st) {
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)k_sample_mlp<K, WPE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sample_mlp<K, WPE>), dim3(grid), dim3(64 * kMlpChains), lds, st, a);
+    hipLaunchKernelGGL((k_sample_mlp<K, WPE>), dim3(grid), dim3(64 * cpb), lds, st, a);
     return hipGetLastError();
 }
 
-// more blocks than CUs: the variant limited to 128 VGPRs when two of its
-// blocks fit a CU (all chains resident at once instead of in rounds of one
-// block per CU); otherwise the unconstrained one
+// Chains per block: 8 in lockstep for the MFMA forward; with the
+// per-wavefront forward the fewest (a power of two) that still give at
+// most one block per CU, so that few chains spread over all CUs instead of
+// sharing SIMDs.  More blocks than CUs: the variant limited to 128 VGPRs
+// when two of its blocks fit a CU (all chains resident at once instead of
+// in rounds of one block per CU); otherwise the unconstrained one.
 template <int K>
 static hipError_t mlp_one(const LearnArgs& a, hipStream_t st) {
     size_t lds = lds_bytes<float>(a.s.d) + mlp_lds_bytes(a.Hn, a.Np, a.s.d.N, a.kmax);
-    int grid = (a.s.n_chains + kMlpChains - 1) / kMlpChains;
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return hipGetLastError();
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return hipErrorInvalidDevice;
+    int cpb = kMlpChains;
+    if (mlp_wave_fwd(a.Hn, a.s.d.N))
+        for (cpb = 1; cpb < kMlpChains && (int64_t)cpb * cus < a.s.n_chains;) cpb *= 2;
+    const int grid = (a.s.n_chains + cpb - 1) / cpb;
     if (grid > cus) {
         if (lds > 65536) {
             hipError_t e = hipFuncSetAttribute((const void*)k_sample_mlp<K, 4>,
@@ -33,10 +39,10 @@ static hipError_t mlp_one(const LearnArgs& a, hipStream_t st) {
             if (e != hipSuccess) return e;
         }
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sample_mlp<K, 4>,
-                                                                    64 * kMlpChains, lds);
+                                                                    64 * cpb, lds);
         if (e != hipSuccess) return e;
     }
-    return per_cu >= 2 ? mlp_launch<K, 4>(a, grid, lds, st) : mlp_launch<K, 1>(a, grid, lds, st);
+    return per_cu >= 2 ? mlp_launch<K, 4>(a, grid, cpb, lds, st) : mlp_launch<K, 1>(a, grid, cpb, lds, st);
 }
 
 hipError_t launch_sample_mlp(const LearnArgs& a, int K, hipStream_t st) {

@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(64 * kMlpChains) __attribute__((amdgpu_waves_p
     Lds<float> l = stage<float>(d, a.t, a.T, a.pb, smem);
     MlpLds m = mlp_stage(la, smem + lds_bytes<float>(d));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int chain = blockIdx.x * kMlpChains + w;
+    const int chain = blockIdx.x * (int)(blockDim.x >> 6) + w;  // blocks of 1..kMlpChains chains
     const bool live = chain < a.n_chains;
     const int N = d.N, Np = la.Np, Hn = la.Hn, kmax = la.kmax;
     int64_t spc = a.n_samples / a.n_chains, res = a.n_samples - spc * a.n_chains;

@@ -97,7 +97,8 @@ def test_learn_logits_matches_oracle(Hn):
     assert np.max(np.abs(z - zo)) <= tol * (1 + np.abs(zo).max()), np.max(np.abs(z - zo))
 
 
-@pytest.mark.parametrize("fraction,scale,chains", [(0.6, 0.8, 256), (1.0, 0.3, 256), (0.6, 0.8, 4096)])
+@pytest.mark.parametrize("fraction,scale,chains", [(0.6, 0.8, 256), (1.0, 0.3, 256), (0.6, 0.8, 4096),
+                                                   (0.6, 0.8, 518)])
 def test_learned_sampler_matches_exact_distribution(fraction, scale, chains):
     """Metropolis-Hastings with the learned cluster proposal (untrained,
     non-uniform network) samples |psi|^2 exactly: <E> against exact
@@ -110,9 +111,11 @@ def test_learned_sampler_matches_exact_distribution(fraction, scale, chains):
     this length reaches |psi|^2 on any device (the exact finite-run
     distribution is 0.65 from it in total variation), while the 0.3-scale
     one relaxes in ~70 slots (test_learned_chain_exact_balance_and_mixing).
-    At fraction 0.6 the plain single flips mix the 0.8-scale chain.  4096
-    chains are 512 blocks of 8, more than the 256 CUs: the launch takes the
-    kernel variant limited to 128 VGPRs (two blocks per CU)."""
+    At fraction 0.6 the plain single flips mix the 0.8-scale chain.  Blocks
+    (this 8 -> 32 -> 8 network is evaluated per wavefront): 256 chains run
+    one per block, 518 four per block (the last block holds two), 4096 are
+    512 blocks of 8, more than the 256 CUs, so the launch takes the kernel
+    variant limited to 128 VGPRs (two blocks per CU)."""
     spec = learn_spec("kitaev", 2, 32, fraction, chains, 32768, warmup=50)
     m, g, rng = _gpu(spec, seed=5, std=0.4)
     o = po.Oracle(spec)
