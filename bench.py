@@ -287,10 +287,14 @@ def learned_ab(epochs_=8, train_epochs=20, target=1e-5, chains=256, spc=256):
         m = rbmgpu.Model(spec)
         g = m.gpu(precision=32)
         g.set_params(params)
-        tr = None
+        tr, tms = None, []
         for e in range(train):
             g.sample_epoch(epoch=1000 + e)
+            g.synchronize()
+            t0 = time.perf_counter()
             tr = g.learn_train(lr=3e-3, beta=0.01, batch=m.n_samples)
+            g.synchronize()
+            tms.append((time.perf_counter() - t0) * 1e3)
         ms, errs, accs, stds = [], [], [], []
         for e in range(epochs_):
             g.sample_epoch(epoch=e)
@@ -305,7 +309,8 @@ def learned_ab(epochs_=8, train_epochs=20, target=1e-5, chains=256, spc=256):
                "binned_std_per_site": float(np.sqrt(np.mean(np.square(stds)))), "acceptance": float(np.mean(accs)),
                "ms_to_target_error": t * (rms / target) ** 2}
         if tr is not None:
-            rec.update({"training_epochs": train, "learned_acceptance": tr["accepted"] / max(tr["proposed"], 1)})
+            rec.update({"training_epochs": train, "learned_acceptance": tr["accepted"] / max(tr["proposed"], 1),
+                        "train_ms_per_step": float(np.median(tms))})
         out[name] = rec
 
     run("plain_bond_flips_0.2", spec(0.2))
@@ -337,11 +342,15 @@ def plain_vs_learned(params, epochs_=3, train_epochs=5, target=1e-4):
         m = rbmgpu.Model(kitaev6_spec(learn=learn))
         g = m.gpu(precision=32)
         g.set_params(params if params is not None else m.initial_weights(421390484))
-        tr = []
+        tr, tms = [], []
         if learn:
             for e in range(train_epochs):
                 g.sample_epoch(epoch=1000 + e)
+                g.synchronize()
+                t0 = time.perf_counter()
                 tr.append(g.learn_train(lr=3e-3, beta=0.01, batch=16384))
+                g.synchronize()
+                tms.append((time.perf_counter() - t0) * 1e3)
         ms, taus, stds, accs, es = [], [], [], [], []
         for e in range(epochs_):
             g.sample_epoch(epoch=e)
@@ -359,7 +368,8 @@ def plain_vs_learned(params, epochs_=3, train_epochs=5, target=1e-4):
                "ms_to_target_error": t * (sd / target) ** 2}
         if learn:
             rec.update({"hidden": learn, "learned_fraction": 0.5, "training_epochs": train_epochs,
-                        "learned_acceptance": tr[-1]["accepted"] / max(tr[-1]["proposed"], 1)})
+                        "learned_acceptance": tr[-1]["accepted"] / max(tr[-1]["proposed"], 1),
+                        "train_ms_per_step": float(np.median(tms))})
         out["learned" if learn else "plain"] = rec
     return out
 

@@ -255,7 +255,8 @@ int rbmgpu_learn_grad(rbmgpu_ctx* ctx, const uint64_t* states, const int32_t* se
                       int64_t n, double beta, double* grads, double* loss);
 /* FP32 master weights in the same layout; set = 1 uploads them */
 int rbmgpu_learn_weights(rbmgpu_ctx* ctx, int32_t set, double* params);
-/* logits of host-given states through the sampler's bf16 MFMA forward */
+/* logits of host-given states through the sampler's own forward (per-wavefront FP32
+   for networks of at most 64 sites and hidden units, else bf16 MFMA) */
 int rbmgpu_learn_logits(rbmgpu_ctx* ctx, const uint64_t* states, int64_t n, double* z);
 /* learn_fraction for the following epochs (0: plain Metropolis) */
 int rbmgpu_learn_set_fraction(rbmgpu_ctx* ctx, double fraction);

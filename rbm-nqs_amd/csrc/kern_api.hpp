@@ -223,7 +223,15 @@ struct TrainArgs {
     float* gb1;
     float* gW2;              // N x Hn
     float* gb2;
+    float* part;             // ks x G partial gradients (G = gW1 .. gb2, contiguous from gW1)
+    int ks;                  // batch slices (train_slices)
+    int64_t kc;              // records per slice
 };
+// batch slices of the gradient reduction: one per 1024 records, at most 256
+__host__ __device__ inline int train_slices(int64_t B) {
+    const int64_t k = (B + 1023) / 1024;
+    return (int)(k < 1 ? 1 : k > 256 ? 256 : k);
+}
 hipError_t launch_train_grad(const TrainArgs& a, hipStream_t st);
 
 // structured (basic RBM) S.x / moments, bf16 MFMA at FP32 accuracy; see kern_struct.hpp

@@ -69,16 +69,24 @@ hipError_t launch_mlp_logits(const LearnArgs& a, const uint64_t* states, int64_t
 }
 
 hipError_t launch_train_grad(const TrainArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_train_fwd, dim3((unsigned)a.B), dim3(512), 0, st, a);
+    // one record per block of the least power of two (>= 64) threads that
+    // covers the hidden units and the sites (the block reductions halve it)
+    int blk = 64;
+    while (blk < a.Hn || blk < a.N) blk *= 2;
+    hipLaunchKernelGGL(k_train_fwd, dim3((unsigned)a.B), dim3(blk), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // gW2 = dZ^T A (N x Hn), gW1 = dU^T S (Hn x N), biases = column sums
-    hipLaunchKernelGGL(k_gemm_tn, dim3((a.Hn + 63) / 64, (a.N + 63) / 64), dim3(256), 0, st, a.dZ, a.N, a.A, a.Hn,
-                       a.gW2, a.N, a.Hn, a.B);
-    hipLaunchKernelGGL(k_gemm_tn, dim3((a.N + 63) / 64, (a.Hn + 63) / 64), dim3(256), 0, st, a.dU, a.Hn, a.S, a.N,
-                       a.gW1, a.Hn, a.N, a.B);
-    hipLaunchKernelGGL(k_colsum_f, dim3((a.Hn + 255) / 256), dim3(256), 0, st, a.dU, a.Hn, a.Hn, a.B, a.gb1);
-    hipLaunchKernelGGL(k_colsum_f, dim3((a.N + 255) / 256), dim3(256), 0, st, a.dZ, a.N, a.N, a.B, a.gb2);
+    // per batch slice: gW2 = dZ^T A (N x Hn) with gb2 = column sums of dZ,
+    // gW1 = dU^T S (Hn x N) with gb1 = column sums of dU, into the slice's
+    // row of part (gW1, gb1, gW2, gb2 as in the gradient); then the sum
+    // over slices in slice order
+    const int64_t G = 2 * (int64_t)a.Hn * a.N + a.Hn + a.N;
+    const int64_t oW1 = 0, ob1 = (int64_t)a.Hn * a.N, oW2 = ob1 + a.Hn, ob2 = oW2 + (int64_t)a.N * a.Hn;
+    hipLaunchKernelGGL(k_gemm_tn_part, dim3((a.Hn + 63) / 64, (a.N + 63) / 64, a.ks), dim3(256), 0, st, a.dZ, a.N,
+                       a.A, a.Hn, a.part + oW2, G, ob2 - oW2, a.N, a.Hn, a.B, a.kc);
+    hipLaunchKernelGGL(k_gemm_tn_part, dim3((a.N + 63) / 64, (a.Hn + 63) / 64, a.ks), dim3(256), 0, st, a.dU, a.Hn,
+                       a.S, a.N, a.part + oW1, G, ob1 - oW1, a.Hn, a.N, a.B, a.kc);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, st, a.part, a.ks, G, G, a.gW1);
     return hipGetLastError();
 }
 

@@ -519,20 +519,25 @@ __global__ void __launch_bounds__(512) k_train_fwd(TrainArgs a) {
     }
 }
 
-// C[m][n] = sum_k X[k][m] Y[k][n] (rows of X, Y are records): 64 x 64 tiles,
-// 256 threads with 4 x 4 outputs each, record chunks of 16 through LDS
-__global__ void __launch_bounds__(256) k_gemm_tn(const float* X, int ldx, const float* Y, int ldy, float* C,
-                                                 int M, int Nn, int64_t K) {
+// One batch slice z (records z kc .. z kc + kc - 1) of C = X^T Y (M x Nn;
+// X: K x M, Y: K x Nn, row-major) into part + z ldz, 64 x 64 tiles through
+// LDS; the blocks of the first column tile also sum X's columns of the
+// slice into part + z ldz + soff (the bias gradients)
+__global__ void __launch_bounds__(256) k_gemm_tn_part(const float* X, int ldx, const float* Y, int ldy, float* part,
+                                                      int64_t ldz, int64_t soff, int M, int Nn, int64_t K,
+                                                      int64_t kc) {
     __shared__ float xs[16][64], ys[16][64];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int64_t kb = (int64_t)blockIdx.z * kc, ke = kb + kc < K ? kb + kc : K;
     float c[4][4] = {};
-    for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    float cs = 0.f;
+    for (int64_t k0 = kb; k0 < ke; k0 += 16) {
         for (int e = threadIdx.x; e < 16 * 64; e += 256) {
             int kk = e / 64, j = e % 64;
             int64_t kr = k0 + kk;
-            xs[kk][j] = (kr < K && m0 + j < M) ? X[kr * ldx + m0 + j] : 0.f;
-            ys[kk][j] = (kr < K && n0 + j < Nn) ? Y[kr * ldy + n0 + j] : 0.f;
+            xs[kk][j] = (kr < ke && m0 + j < M) ? X[kr * ldx + m0 + j] : 0.f;
+            ys[kk][j] = (kr < ke && n0 + j < Nn) ? Y[kr * ldy + n0 + j] : 0.f;
         }
         __syncthreads();
 #pragma unroll
@@ -542,8 +547,12 @@ __global__ void __launch_bounds__(256) k_gemm_tn(const float* X, int ldx, const 
 #pragma unroll
                 for (int j = 0; j < 4; j++) c[i][j] += xs[kk][ty * 4 + i] * ys[kk][tx * 4 + j];
         }
+        if (blockIdx.x == 0 && threadIdx.x < 64)
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) cs += xs[kk][threadIdx.x];
         __syncthreads();
     }
+    float* C = part + (int64_t)blockIdx.z * ldz;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -551,15 +560,16 @@ __global__ void __launch_bounds__(256) k_gemm_tn(const float* X, int ldx, const 
             int mm = m0 + ty * 4 + i, nn = n0 + tx * 4 + j;
             if (mm < M && nn < Nn) C[(int64_t)mm * Nn + nn] = c[i][j];
         }
+    if (blockIdx.x == 0 && threadIdx.x < 64 && m0 + (int)threadIdx.x < M) C[soff + m0 + threadIdx.x] = cs;
 }
 
-// column sums: out[j] = sum_k X[k][j]
-__global__ void __launch_bounds__(256) k_colsum_f(const float* X, int ld, int n, int64_t K, float* out) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    float s = 0;
-    for (int64_t k = 0; k < K; k++) s += X[k * ld + j];
-    out[j] = s;
+// out[i] = sum over the ks slices of part[z ldz + i], in slice order
+__global__ void __launch_bounds__(256) k_sum_slices(const float* part, int ks, int64_t ldz, int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < ks; z++) s += part[(int64_t)z * ldz + i];
+    out[i] = s;
 }
 
 }  // namespace rbmgpu

@@ -145,6 +145,7 @@ struct rbmgpu_ctx {
     uint64_t* d_tst = nullptr;
     int16_t* d_tseed = nullptr;
     float* d_tw = nullptr;
+    float* d_tpart = nullptr;  // per batch slice partial gradients
     bool learned_epoch = false;  // the last epoch ran the learned sampler
     int64_t mlp_n() const { return (int64_t)Hn * N + Hn + (int64_t)N * Hn + N; }
     // device address of the static scratch array on this context's device
@@ -508,6 +509,7 @@ int ensure_train(rbmgpu_ctx* c, int64_t B) {
     if (int rc = dalloc(ctx, &c->d_tst, (size_t)B * c->nw * sizeof(uint64_t))) return rc;
     if (int rc = dalloc(ctx, &c->d_tseed, (size_t)B * sizeof(int16_t))) return rc;
     if (int rc = dalloc(ctx, &c->d_tw, (size_t)B * sizeof(float))) return rc;
+    if (int rc = dalloc(ctx, &c->d_tpart, (size_t)train_slices(B) * c->mlp_n() * sizeof(float))) return rc;
     c->tr_cap = B;
     return 0;
 }
@@ -539,6 +541,9 @@ int learn_grad(rbmgpu_ctx* c, int64_t B, double beta, double* loss) {
     t.gb1 = t.gW1 + (int64_t)c->Hn * c->N;
     t.gW2 = t.gb1 + c->Hn;
     t.gb2 = t.gW2 + (int64_t)c->N * c->Hn;
+    t.part = c->d_tpart;
+    t.ks = train_slices(B);
+    t.kc = (B + t.ks - 1) / t.ks;
     HIPCK(launch_train_grad(t, c->stream));
     std::vector<float> lv(B);
     HIPCK(hipMemcpyAsync(lv.data(), t.loss, B * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -2287,7 +2292,7 @@ void rbmgpu_destroy(rbmgpu_ctx* c) {
                     c->d_Opf,    c->d_pdot,   c->d_cyc,    c->d_seedc, c->d_seedl,  c->d_W1b,    c->d_W2b,
                     c->d_b1p,    c->d_b2p,    c->d_mlp,    c->d_adam,  c->d_grad,   c->d_rec_st, c->d_rec_seed,
                     c->d_rec_k,  c->d_rec_a,  c->d_lacc,   c->d_lprop, c->d_tr,     c->d_tst,    c->d_tseed,
-                    c->d_tw,     c->d_ratio_term, c->d_term_slot, c->d_params, c->d_par, c->d_stats, c->d_Smat, c->d_gpart, c->d_w, c->d_wsx, c->d_lp,
+                    c->d_tw,     c->d_tpart,     c->d_ratio_term, c->d_term_slot, c->d_params, c->d_par, c->d_stats, c->d_Smat, c->d_gpart, c->d_w, c->d_wsx, c->d_lp,
                     c->d_wred, c->d_pmap, c->d_pairs, c->d_tbond};
     drop_graphs(c);
     for (void* p : ptrs)

@@ -57,14 +57,15 @@ def _gpu(spec, seed=0, std=0.2):
     return m, g, rng
 
 
-@pytest.mark.parametrize("beta", [0.0, 0.05])
-def test_learn_grad_matches_oracle(beta):
+@pytest.mark.parametrize("beta,n", [(0.0, 40), (0.05, 40), (0.05, 5000)])
+def test_learn_grad_matches_oracle(beta, n):
+    """the FP32 batch gradient against the FP64 oracle; 5000 records are 5
+    batch slices of 1000 summed in slice order"""
     m, g, rng = _gpu(learn_spec("kitaev", 3, 48, 0.5, 4, 64))
     N, Hn = m.N, 48
     net = _net(rng, N, Hn)
     g.learn_weights(_flat(*net))
     net32 = [np.asarray(x, dtype=np.float32).astype(np.float64) for x in net]
-    n = 40
     st = po.random_states(rng, n, N)
     seeds = rng.integers(0, N, n)
     seeds[3] = -1  # a record without a learned proposal contributes nothing
