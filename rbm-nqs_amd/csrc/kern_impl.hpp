@@ -435,7 +435,8 @@ __device__ inline void eval_obs_t(const Dims& d, const Lds<R>& l, const Units<R,
 // (interleaved butterflies) and multiplied by the visible-bias factor
 // exp(-2 sum sigma_f c_f), computed with v_exp_f32 and sincos_f.  A term
 // with more than one off-diagonal entry finishes the others one by one.
-// Used for K <= 2 units per lane, term tables in LDS and the basic RBM.
+// Used for K <= 2 units per lane and term tables in LDS (basic and
+// symmetric RBM).
 constexpr int kObsWalk = 4, kObsBatch = 4;
 
 template <int L, int B, int O = 1>
@@ -624,10 +625,14 @@ template <typename R, int L, int K, bool PF>
 __device__ inline void eval_obs(const Dims& d, const Lds<R>& l, const Units<R, L, K>& U,
                                 const cplx<R> (&pv)[16], const uint64_t* st, int ln, double* out) {
     // (with more units per lane the batch would not fit in the registers;
-    // one inlined copy only: term tables in LDS, one table row per site)
+    // term tables in LDS; the basic RBM reads table row f for site f, the
+    // symmetric one row P[s][f] of the unit's symmetry image s)
     if constexpr (sizeof(R) == 4 && !PF && K <= 2) {
-        if (d.n_terms <= kObsWalk * L && d.t_lds && d.n_symm == 1) {
-            eval_obs_batch<L, K, true, true>(d, l, U, pv, st, ln, out);
+        if (d.n_terms <= kObsWalk * L && d.t_lds) {
+            if (d.n_symm == 1)
+                eval_obs_batch<L, K, true, true>(d, l, U, pv, st, ln, out);
+            else
+                eval_obs_batch<L, K, true, false>(d, l, U, pv, st, ln, out);
             return;
         }
     }

@@ -143,6 +143,21 @@ def test_learned_sampler_matches_exact_distribution(fraction, scale, chains):
     assert tv < 0.04, tv
 
 
+def test_learned_sampler_symmetric_local_energies():
+    """the learned sampler on a symmetric RBM (Kitaev 2x2, alpha 4): the
+    local energies of its kept states against the oracle, 1e-4 of the
+    energy scale"""
+    spec = learn_spec("kitaev", 2, 32, 0.5, 64, 2048, alpha=4, rbm="symmetry", warmup=20)
+    m, g, rng = _gpu(spec, seed=9, std=0.1)
+    o = po.Oracle(spec)
+    o.set_params(random_params(np.random.default_rng(9), m.P, 0.1))
+    g.learn_weights(_flat(*_net(rng, m.N, 32)))
+    g.sample_epoch(epoch=0)
+    st, obs, _, _ = g.get_samples(states=True, obs=True, O=False)
+    e_ref = o.eloc(st)
+    assert np.max(np.abs(obs[:, 0] - e_ref)) <= 1e-4 * (np.abs(e_ref).max() + 1)
+
+
 def test_learned_sampler_trains():
     """sampling -> records -> REINFORCE/Adam steps: finite loss, weights
     move, learned moves are proposed and accepted"""

@@ -400,3 +400,28 @@ sr.max_iterations = 50
     p_or = o.get_params()
     # 50 CG iterations amplify FP64 rounding differences to ~1e-8 relative
     assert np.max(np.abs(p_gpu - p_or)) <= 1e-7 * np.abs(p_or - p0).max()
+
+
+@pytest.mark.parametrize("model,n,alpha,lanes", [("kitaev", 2, 4, 16), ("kitaev", 2, 4, 8), ("toric", 3, 2, 32)])
+def test_group_sampler_symmetric_local_energies(model, n, alpha, lanes):
+    """the lane-group FP32 sampler's batched local energies on symmetric
+    RBMs (each unit reads the table row P[s][f] of its symmetry image s)
+    against the oracle on its kept states, 1e-4 of the energy scale:
+    Kitaev 2x2 alpha 4 (16 units; 16 lanes = one unit per lane, 8 lanes =
+    two) and toric 3x3 alpha 2 (18 units on 32 lanes, 4-site terms)"""
+    spec = ini(model, n, rbm="symmetry", alpha=alpha,
+               extra="[sampler]\nn_samples = 512\nmetropolis.n_chains = 32\nmetropolis.n_warmup_steps = 10\n")
+    print(f"[gpu-test] group sampler symmetric {model}{n} alpha={alpha} lanes={lanes}", flush=True)
+    rng = np.random.default_rng(8)
+    m = rbmgpu.Model(spec)
+    o = po.Oracle(spec)
+    p = random_params(rng, m.P, 0.1)
+    o.set_params(p)
+    g = m.gpu(precision=32, lanes=lanes)
+    g.set_params(p)
+    g.sample_epoch(epoch=0)
+    assert not g.wave_sampler() and g.launch_shape()[0] == lanes, g.launch_shape()
+    st, obs, _, _ = g.get_samples(states=True, obs=True, O=False)
+    e_ref = o.eloc(st)
+    assert np.max(np.abs(obs[:, 0] - e_ref)) <= 1e-4 * (np.abs(e_ref).max() + 1)
+    assert len({tuple(r) for r in st}) > m.n_chains
