@@ -196,9 +196,12 @@ __host__ __device__ inline size_t mlp_lds_bytes(int Hn, int Np, int N, int kmax)
     if (mlp_wave_fwd(Hn, N)) b += 2 * (size_t)Hn * Np * 4 + 16 * (size_t)Hn * 4;  // W1^T, W2^T FP32, hidden rows
     return (b + 15) & ~(size_t)15;
 }
-// chains (wavefronts) per k_sample_mlp block; the MFMA tiles have 16 rows,
-// rows kMlpChains..15 stay zero
+// chains (wavefronts) per k_sample_mlp block: at most kMlpChains for the
+// unconstrained kernel (two waves per SIMD at its register count), up to
+// kMlpChainsMax (the 16 rows of the MFMA tiles) for the 128-VGPR variant;
+// MFMA rows beyond the block's chains stay zero
 constexpr int kMlpChains = 8;
+constexpr int kMlpChainsMax = 16;
 hipError_t launch_sample_mlp(const LearnArgs& a, int K, hipStream_t st);
 hipError_t launch_mlp_logits(const LearnArgs& a, const uint64_t* states, int64_t n, float* z, hipStream_t st);
 

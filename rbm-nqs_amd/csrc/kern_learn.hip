@@ -20,16 +20,19 @@ static hipError_t mlp_launch(const LearnArgs& a, int grid, int cpb, size_t lds, 
 // most one block per CU, so that few chains spread over all CUs instead of
 // sharing SIMDs.  More blocks than CUs: the variant limited to 128 VGPRs
 // when two of its blocks fit a CU (all chains resident at once instead of
-// in rounds of one block per CU); otherwise the unconstrained one.
+// in rounds of one block per CU); for the MFMA forward, when only one
+// block fits (the weights in LDS), that variant with 16 chains per block
+// (the full MFMA tile, half the rounds); otherwise the unconstrained one.
 template <int K>
 static hipError_t mlp_one(const LearnArgs& a, hipStream_t st) {
     size_t lds = lds_bytes<float>(a.s.d) + mlp_lds_bytes(a.Hn, a.Np, a.s.d.N, a.kmax);
-    int dev = 0, cus = 0, per_cu = 0;
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
         return hipErrorInvalidDevice;
+    const bool wf = mlp_wave_fwd(a.Hn, a.s.d.N);
     int cpb = kMlpChains;
-    if (mlp_wave_fwd(a.Hn, a.s.d.N))
+    if (wf)
         for (cpb = 1; cpb < kMlpChains && (int64_t)cpb * cus < a.s.n_chains;) cpb *= 2;
     const int grid = (a.s.n_chains + cpb - 1) / cpb;
     if (grid > cus) {
@@ -38,11 +41,21 @@ static hipError_t mlp_one(const LearnArgs& a, hipStream_t st) {
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
         }
+        int per_cu = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sample_mlp<K, 4>,
                                                                     64 * cpb, lds);
         if (e != hipSuccess) return e;
+        if (per_cu >= 2) return mlp_launch<K, 4>(a, grid, cpb, lds, st);
+        if (!wf) {
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sample_mlp<K, 4>,
+                                                             64 * kMlpChainsMax, lds);
+            if (e != hipSuccess) return e;
+            if (per_cu >= 1)
+                return mlp_launch<K, 4>(a, (a.s.n_chains + kMlpChainsMax - 1) / kMlpChainsMax, kMlpChainsMax, lds,
+                                        st);
+        }
     }
-    return per_cu >= 2 ? mlp_launch<K, 4>(a, grid, cpb, lds, st) : mlp_launch<K, 1>(a, grid, cpb, lds, st);
+    return mlp_launch<K, 1>(a, grid, cpb, lds, st);
 }
 
 hipError_t launch_sample_mlp(const LearnArgs& a, int K, hipStream_t st) {

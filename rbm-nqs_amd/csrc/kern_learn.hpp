@@ -106,11 +106,12 @@ __device__ inline MlpLds mlp_stage(const LearnArgs& a, char* base) {
 }
 
 // Zt = W2 relu(W1 Sg + b1) + b2 for the 16 rows of Sg (all waves of the
-// block take part; syncs included)
+// block take part, 16-column tiles dealt round-robin; syncs included)
 __device__ inline void mlp_forward16(const MlpLds& m, int Hn, int Np) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    const int nwv = (int)(blockDim.x >> 6);
     __syncthreads();  // Sg rows written
-    for (int ct = w; ct < Hn / 16; ct += kMlpChains) {
+    for (int ct = w; ct < Hn / 16; ct += nwv) {
         v4f_l acc = {0, 0, 0, 0};
         for (int kb = 0; kb < Np / 16; kb++) {
             v4s av = *reinterpret_cast<const v4s*>(m.Sg + lr * Np + 16 * kb + 4 * lq);
@@ -122,7 +123,7 @@ __device__ inline void mlp_forward16(const MlpLds& m, int Hn, int Np) {
         for (int r = 0; r < 4; r++) m.Ha[(4 * lq + r) * Hn + h] = to_bf16(fmaxf(acc[r] + m.b1[h], 0.f));
     }
     __syncthreads();
-    for (int ct = w; ct < Np / 16; ct += kMlpChains) {
+    for (int ct = w; ct < Np / 16; ct += nwv) {
         v4f_l acc = {0, 0, 0, 0};
         for (int kb = 0; kb < Hn / 16; kb++) {
             v4s av = *reinterpret_cast<const v4s*>(m.Ha + lr * Hn + 16 * kb + 4 * lq);
@@ -198,17 +199,19 @@ __device__ inline float seed_mass(const MlpLds& m, const float* z, float lse, in
 }
 
 // WPE: minimum waves per SIMD the register allocation must allow (1: no
-// limit, the fastest code for one block per CU; 4: at most 128 VGPRs, so
-// that two blocks share a CU when there are more blocks than CUs)
+// limit, the fastest code for one block per CU, blocks of at most
+// kMlpChains chains; 4: at most 128 VGPRs, so that two blocks of 8 or one
+// of 16 chains share a CU)
 template <int K, int WPE>
-__global__ void __launch_bounds__(64 * kMlpChains) __attribute__((amdgpu_waves_per_eu(WPE))) k_sample_mlp(LearnArgs la) {
+__global__ void __launch_bounds__(64 * (WPE >= 4 ? kMlpChainsMax : kMlpChains))
+    __attribute__((amdgpu_waves_per_eu(WPE))) k_sample_mlp(LearnArgs la) {
     extern __shared__ __align__(16) char smem[];
     const SampArgs<float>& a = la.s;
     const Dims d = a.d;
     Lds<float> l = stage<float>(d, a.t, a.T, a.pb, smem);
     MlpLds m = mlp_stage(la, smem + lds_bytes<float>(d));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int chain = blockIdx.x * (int)(blockDim.x >> 6) + w;  // blocks of 1..kMlpChains chains
+    const int chain = blockIdx.x * (int)(blockDim.x >> 6) + w;  // blocks of 1..kMlpChainsMax chains
     const bool live = chain < a.n_chains;
     const int N = d.N, Np = la.Np, Hn = la.Hn, kmax = la.kmax;
     int64_t spc = a.n_samples / a.n_chains, res = a.n_samples - spc * a.n_chains;

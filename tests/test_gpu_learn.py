@@ -143,6 +143,31 @@ def test_learned_sampler_matches_exact_distribution(fraction, scale, chains):
     assert tv < 0.04, tv
 
 
+def test_learned_sampler_mfma_16_chain_blocks_exact_energy():
+    """the MFMA forward in blocks of 16 chains (the full 16-row tile): a
+    18 -> 512 -> 18 network keeps ~100 KB of LDS per block, so one block
+    fits a CU, and 4096 chains (512 blocks of 8, more than the 256 CUs)
+    run as 256 blocks of 16 in the 128-VGPR variant; <E> of Kitaev 3x3
+    against exact enumeration (2^18 states) within 5 binned standard
+    errors"""
+    spec = learn_spec("kitaev", 3, 512, 0.6, 4096, 32768, warmup=50)
+    m, g, rng = _gpu(spec, seed=11, std=0.3)
+    o = po.Oracle(spec)
+    o.set_params(random_params(np.random.default_rng(11), m.P, 0.3))
+    g.learn_weights(_flat(*_net(rng, m.N, 512, 0.4, 0.03)))
+    exact = o.exact_expectation()
+    means, stds = [], []
+    for ep in range(3):
+        g.sample_epoch(epoch=ep)
+        r = g.reduce(want_o=False)
+        means.append(r["mean"].real)
+        stds.append(r["std"])
+    est = np.mean(means)
+    err = np.sqrt(np.mean(np.square(stds)) / len(stds))
+    print(f"[gpu-test] 16-chain MFMA blocks: <E> {est} +- {err} (exact {exact.real})", flush=True)
+    assert abs(est - exact.real) < 5 * err + 1e-4, (est, exact, err)
+
+
 def test_learned_sampler_symmetric_local_energies():
     """the learned sampler on a symmetric RBM (Kitaev 2x2, alpha 4): the
     local energies of its kept states against the oracle, 1e-4 of the
