@@ -306,7 +306,7 @@ def test_learned_beats_single_flips_on_converged_state():
     training epochs on the GPU) mix the chains, so at equal sample counts the
     rms error of E/N against the exact expectation value of the same weights
     (GPU exact enumeration, FP64) is smaller.  Both runs are deterministic
-    (Philox streams); the measured ratio is about 0.4, the bound 0.8."""
+    (Philox streams); the measured ratio is about 0.5, the bound 0.8."""
     import os
     import sys
 
