@@ -152,6 +152,7 @@ __device__ inline void mlp_forward_wave(const MlpLds& m, int Hn, int Np, int N, 
     float* hw = m.Hw + w * Hn;
     if (lane < Hn) {
         float acc = m.b1[lane];
+#pragma unroll 8
         for (int i = 0; i < N; i++) {
             const int b = bitof(st, i) ^ (int)(((i < 64 ? f0 : f1) >> (i & 63)) & 1);
             const float wv = m.W1t[i * Hn + lane];
@@ -319,21 +320,29 @@ __global__ void __launch_bounds__(64 * (WPE >= 4 ? kMlpChainsMax : kMlpChains))
                 // seed i ~ softmax(z(sigma)): inclusive scans of the lanes'
                 // probabilities (sites lane and lane + 64), first crossing of us
                 const float* zc = m.Zc + w * Np;
-                float p0 = lane < N ? __expf(zc[lane] - lse) : 0.f;
-                float p1 = lane + 64 < N ? __expf(zc[lane + 64] - lse) : 0.f;
-                float c0 = p0, c1 = p1;
+                // (sites 64.. only when the first 64 do not cross us)
+                float c0 = lane < N ? __expf(zc[lane] - lse) : 0.f;
 #pragma unroll
                 for (int o = 1; o < 64; o <<= 1) {
-                    float t0 = __shfl_up(c0, o), t1 = __shfl_up(c1, o);
-                    if (lane >= o) {
-                        c0 += t0;
-                        c1 += t1;
-                    }
+                    float t0 = __shfl_up(c0, o);
+                    if (lane >= o) c0 += t0;
                 }
-                float tot0 = __shfl(c0, 63);
-                uint64_t b0 = __ballot(lane < N && c0 > us);
-                uint64_t b1 = __ballot(lane + 64 < N && tot0 + c1 > us);
-                seed = b0 ? __ffsll((long long)b0) - 1 : (b1 ? 64 + __ffsll((long long)b1) - 1 : N - 1);
+                const uint64_t b0 = __ballot(lane < N && c0 > us);
+                if (b0) {
+                    seed = __ffsll((long long)b0) - 1;
+                } else if (N > 64) {
+                    float c1 = lane + 64 < N ? __expf(zc[lane + 64] - lse) : 0.f;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        float t1 = __shfl_up(c1, o);
+                        if (lane >= o) c1 += t1;
+                    }
+                    const float tot0 = __shfl(c0, 63);
+                    const uint64_t b1 = __ballot(lane + 64 < N && tot0 + c1 > us);
+                    seed = b1 ? 64 + __ffsll((long long)b1) - 1 : N - 1;
+                } else {
+                    seed = N - 1;
+                }
                 kk = (code >> 16) & 0xFF;
 #pragma unroll
                 for (int t = 0; t < 8; t++) {
