@@ -148,6 +148,24 @@ def test_driver_optimizer_family(tmp_path, opt):
     assert rows[-10:, 0].mean() < rows[:10, 0].mean() - 0.01, (opt, rows[:10, 0].mean(), rows[-10:, 0].mean())
 
 
+def test_driver_learned_proposal_trains(tmp_path):
+    """`learn.hidden` switches on the learned cluster proposal in the driver:
+    every epoch samples with learned moves (fraction 0.5) and runs one
+    REINFORCE / Adam step of the network on the epoch's records; the SR
+    run still descends (same bounds as the optimizer family)"""
+    d = str(tmp_path)
+    text = INI.format(name="l", epochs=60) + "[learn]\nhidden = 32\nfraction = 0.5\nbatch = 2000\n"
+    with open(os.path.join(d, "run.ini"), "w") as f:
+        f.write(text)
+    rc, out = _run(["-i", "run.ini", "--train"], d)
+    assert rc == 0, out
+    log = open(os.path.join(d, "l.log")).read().splitlines()
+    rows = np.array([[float(x) for x in l.split(",")] for l in log[1:]])
+    assert rows.shape[0] == 60 and np.all(np.isfinite(rows))
+    assert np.all(rows[:, 0] > -0.8660254037844461 - 5 * rows[:, 1].max())
+    assert rows[-10:, 0].mean() < rows[:10, 0].mean() - 0.01, (rows[:10, 0].mean(), rows[-10:, 0].mean())
+
+
 @pytest.mark.parametrize("extra", ["", "[optimizer]\nplugin = momentum\n"])
 def test_driver_massive_step_exits_55(tmp_path, extra):
     """a checkpoint with a NaN weight: every local energy, F and step is NaN,
